@@ -1,0 +1,104 @@
+"""ctypes binding of libhdbscan_mi355x.so (the gfx950 C ABI).
+
+Loading fails loudly: there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libhdbscan_mi355x.so"
+
+HDB_OK, HDB_EINVAL, HDB_ENOMEM, HDB_EDEVICE, HDB_EDISCONNECTED, HDB_EUNSUPPORTED = range(6)
+STATUS_NAMES = {0: "HDB_OK", 1: "HDB_EINVAL", 2: "HDB_ENOMEM", 3: "HDB_EDEVICE", 4: "HDB_EDISCONNECTED",
+                5: "HDB_EUNSUPPORTED"}
+
+# NNDElement (tools/nndescent_element.cuh:9-57) and MSTedge
+# (HDBSCAN/structs/hdbscan_elements.cuh:40-45) as numpy record types.
+NND_ELEMENT = np.dtype([("distance", "<f4"), ("label", "<i4")])
+MST_EDGE = np.dtype([("from_node", "<i4"), ("to_node", "<i4"), ("weight", "<f4")])
+
+
+class HdbParams(C.Structure):
+    _fields_ = [
+        ("k", C.c_int32),
+        ("sample_num", C.c_int32),
+        ("nnd_iteration", C.c_int32),
+        ("merge_sample_num", C.c_int32),
+        ("merge_iteration", C.c_int32),
+        ("mpts", C.c_int32),
+        ("distance_mode", C.c_int32),
+        ("rerank_keep", C.c_int32),
+        ("bucket_cap", C.c_int32),
+        ("reserved", C.c_int32 * 7),
+    ]
+
+
+class HdbPipelineOpts(C.Structure):
+    _fields_ = [
+        ("knng_mode", C.c_int32),
+        ("blocks", C.c_int32),
+        ("keep_csr", C.c_int32),
+        ("reserved", C.c_int32 * 5),
+    ]
+
+
+VP = C.c_void_p
+I32 = C.c_int32
+I64 = C.c_int64
+PI64 = C.POINTER(C.c_int64)
+PI32 = C.POINTER(C.c_int32)
+
+# name -> (restype, argtypes); the same list is what include/hdbscan_mi355x.h declares
+PROTOTYPES = {
+    "hdb_params_default": (None, [C.POINTER(HdbParams)]),
+    "hdb_ctx_create": (C.c_int, [C.POINTER(VP), C.c_int]),
+    "hdb_ctx_destroy": (C.c_int, [VP]),
+    "hdb_last_error": (C.c_char_p, [VP]),
+    "hdb_synchronize": (C.c_int, [VP]),
+    "hdb_dev_alloc": (C.c_int, [VP, C.POINTER(VP), C.c_size_t]),
+    "hdb_dev_free": (C.c_int, [VP, VP]),
+    "hdb_memcpy_h2d": (C.c_int, [VP, VP, VP, C.c_size_t]),
+    "hdb_memcpy_d2h": (C.c_int, [VP, VP, VP, C.c_size_t]),
+    "hdb_memcpy_d2d": (C.c_int, [VP, VP, VP, C.c_size_t]),
+    "hdb_timer_enable": (C.c_int, [VP, C.c_int]),
+    "hdb_timer_reset": (C.c_int, [VP]),
+    "hdb_timer_read": (C.c_int, [VP, C.POINTER(C.c_char_p), C.POINTER(C.c_double), PI64, C.c_int,
+                                 C.POINTER(C.c_int)]),
+    "hdb_nnd_stats": (C.c_int, [VP, PI64, PI64, PI64]),
+    "hdb_nndescent": (C.c_int, [VP, VP, I64, I32, C.POINTER(HdbParams), VP]),
+    "hdb_nndescent_from": (C.c_int, [VP, VP, I64, I32, C.POINTER(HdbParams), I32, VP]),
+    "hdb_knn_merge": (C.c_int, [VP, VP, I64, I64, I32, VP, VP, C.POINTER(HdbParams), VP]),
+    "hdb_build_knng_blocked": (C.c_int, [VP, VP, I64, I32, I32, C.POINTER(HdbParams), VP]),
+    "hdb_bruteforce_knn": (C.c_int, [VP, VP, I64, I32, VP, I64, I32, VP, VP]),
+    "hdb_hub_counts": (C.c_int, [VP, VP, I64, I32, VP]),
+    "hdb_antihubs": (C.c_int, [VP, VP, VP, I64, I32, VP, PI64]),
+    "hdb_core_distances": (C.c_int, [VP, VP, VP, I64, I32, I32, I32, VP]),
+    "hdb_mrd_graph": (C.c_int, [VP, VP, VP, VP, VP, I64, I64, I32, I32, VP, VP, VP, PI64]),
+    "hdb_mst": (C.c_int, [VP, VP, VP, VP, I64, VP, PI64, PI64]),
+    "hdb_sort_edges": (C.c_int, [VP, VP, I64]),
+    "hdb_pipeline": (C.c_int, [VP, VP, I64, I32, C.POINTER(HdbParams), C.POINTER(HdbPipelineOpts), VP, VP, VP,
+                               VP, PI64, PI64, PI64]),
+    "hdb_labels": (C.c_int, [VP, I64, I32, VP, PI32]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load the gfx950 library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m gpu_hdbscan_amd.build` first "
+                               "(there is no CPU fallback)")
+        lib = C.CDLL(str(LIB_PATH), mode=os.RTLD_NOW | getattr(os, "RTLD_LOCAL", 0))
+        for name, (res, args) in PROTOTYPES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib

@@ -1,0 +1,348 @@
+"""Host-side mirror of the reference's interface for the hot path.
+
+Functions keep the reference's names and argument meaning
+(gpuknn::NNDescent, gpuknn::KNNMergeFromHost, GenLargeKNNGraph,
+calculate_degrees, calculateCoreDistance, buildEnhancedKNNG, gpuMST/buildMST,
+sort_edges, get_clusters); each runs on the gfx950 library through the C ABI.
+Errors raise HdbError instead of exiting the process (the reference calls
+exit(-1) on CUDA errors, e.g. gpuknn/nndescent.cu:637-642).
+
+Arrays may be numpy host arrays (uploaded, results downloaded) or
+DeviceArray objects (resident in HBM; used as is).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import MST_EDGE, NND_ELEMENT, HdbParams, HdbPipelineOpts
+
+K = 32
+
+
+class HdbError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{_lib.STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+def default_params(**overrides) -> HdbParams:
+    p = HdbParams()
+    _lib.load().hdb_params_default(C.byref(p))
+    for k, v in overrides.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+class DeviceArray:
+    """A device allocation owned by a Context, with a numpy-like shape/dtype."""
+
+    def __init__(self, ctx: "Context", shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        ptr = C.c_void_p()
+        ctx._check(ctx.lib.hdb_dev_alloc(ctx.handle, C.byref(ptr), max(self.nbytes, 1)))
+        self.ptr = ptr.value
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            self.ctx._check(self.ctx.lib.hdb_memcpy_d2h(self.ctx.handle, out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def upload(self, arr: np.ndarray) -> "DeviceArray":
+        arr = np.ascontiguousarray(arr, dtype=self.dtype)
+        if arr.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        if self.nbytes:
+            self.ctx._check(self.ctx.lib.hdb_memcpy_h2d(self.ctx.handle, self.ptr, arr.ctypes.data, self.nbytes))
+        return self
+
+    def free(self) -> None:
+        if self.ptr:
+            self.ctx.lib.hdb_dev_free(self.ctx.handle, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx.handle:
+                self.ctx.lib.hdb_dev_free(self.ctx.handle, self.ptr)
+        except Exception:
+            pass
+
+
+class Context:
+    """One hdb_ctx: a device, a HIP stream and a scratch arena."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _lib.load()
+        h = C.c_void_p()
+        st = self.lib.hdb_ctx_create(C.byref(h), int(device))
+        if st != _lib.HDB_OK:
+            raise HdbError(st, f"hdb_ctx_create(device={device}) failed")
+        self.handle = h.value
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.hdb_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, st: int, allow=()) -> int:
+        if st != _lib.HDB_OK and st not in allow:
+            msg = self.lib.hdb_last_error(self.handle)
+            raise HdbError(st, msg.decode() if msg else "")
+        return st
+
+    # -- memory -------------------------------------------------------------
+    def empty(self, shape, dtype) -> DeviceArray:
+        return DeviceArray(self, shape, dtype)
+
+    def to_device(self, arr: np.ndarray, dtype=None) -> DeviceArray:
+        arr = np.ascontiguousarray(arr, dtype=dtype or arr.dtype)
+        return DeviceArray(self, arr.shape, arr.dtype).upload(arr)
+
+    def dev(self, x, dtype=None) -> DeviceArray:
+        return x if isinstance(x, DeviceArray) else self.to_device(np.asarray(x), dtype)
+
+    def synchronize(self) -> None:
+        self._check(self.lib.hdb_synchronize(self.handle))
+
+    # -- timers ---------------------------------------------------------------
+    def timer_enable(self, on: bool = True) -> None:
+        self._check(self.lib.hdb_timer_enable(self.handle, 1 if on else 0))
+
+    def timer_reset(self) -> None:
+        self._check(self.lib.hdb_timer_reset(self.handle))
+
+    def timer_read(self) -> dict:
+        cap = 128
+        names = (C.c_char_p * cap)()
+        ms = (C.c_double * cap)()
+        cnt = (C.c_int64 * cap)()
+        n = C.c_int()
+        self._check(self.lib.hdb_timer_read(self.handle, names, ms, cnt, cap, C.byref(n)))
+        return {names[i].decode(): (ms[i], cnt[i]) for i in range(min(n.value, cap))}
+
+    def nnd_stats(self) -> dict:
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        self._check(self.lib.hdb_nnd_stats(self.handle, C.byref(a), C.byref(b), C.byref(c)))
+        return {"proposals": a.value, "bucket_drops": b.value, "reverse_drops": c.value}
+
+
+def _as_points(ctx: Context, vectors) -> tuple[DeviceArray, int, int]:
+    X = ctx.dev(vectors, np.float32)
+    if len(X.shape) != 2:
+        raise ValueError("vectors must be n x d")
+    return X, X.shape[0], X.shape[1]
+
+
+def _out(ctx, like, dev_out, shape, dtype):
+    return ctx.empty(shape, dtype)
+
+
+def _ret(arr: DeviceArray, keep_on_device: bool):
+    return arr if keep_on_device else arr.numpy()
+
+
+# --------------------------------------------------------------- gpuknn --
+def NNDescent(ctx: Context, vectors, iteration: int = 6, params: HdbParams | None = None,
+              store_result_in_device: bool = False):
+    """gpuknn::NNDescent (gpuknn/nndescent.cuh:40-42): n x 32 NNDElement lists."""
+    X, n, d = _as_points(ctx, vectors)
+    p = params or default_params()
+    p.nnd_iteration = int(iteration)
+    out = ctx.empty((n, K), NND_ELEMENT)
+    ctx._check(ctx.lib.hdb_nndescent(ctx.handle, X.ptr, n, d, C.byref(p), out.ptr))
+    return _ret(out, store_result_in_device)
+
+
+def NNDescentFrom(ctx: Context, vectors, lists, iterations: int, params: HdbParams | None = None,
+                  store_result_in_device: bool = False):
+    """NN-Descent iterations from given initial lists (learned init entry)."""
+    X, n, d = _as_points(ctx, vectors)
+    p = params or default_params()
+    L = ctx.to_device(np.ascontiguousarray(lists, dtype=NND_ELEMENT)) if not isinstance(lists, DeviceArray) else lists
+    ctx._check(ctx.lib.hdb_nndescent_from(ctx.handle, X.ptr, n, d, C.byref(p), int(iterations), L.ptr))
+    return _ret(L, store_result_in_device)
+
+
+def KNNMergeFromHost(ctx: Context, vectors_first, knngraph_first, vectors_second, knngraph_second,
+                     params: HdbParams | None = None, store_result_in_device: bool = False):
+    """gpuknn::KNNMergeFromHost (gpuknn/knnmerge.cu:478-530)."""
+    v1 = np.ascontiguousarray(vectors_first, np.float32)
+    v2 = np.ascontiguousarray(vectors_second, np.float32)
+    n1, n2, d = v1.shape[0], v2.shape[0], v1.shape[1]
+    X = ctx.to_device(np.concatenate([v1, v2], axis=0))
+    g1 = ctx.dev(np.ascontiguousarray(knngraph_first, NND_ELEMENT))
+    g2 = ctx.dev(np.ascontiguousarray(knngraph_second, NND_ELEMENT))
+    p = params or default_params()
+    out = ctx.empty((n1 + n2, K), NND_ELEMENT)
+    ctx._check(ctx.lib.hdb_knn_merge(ctx.handle, X.ptr, n1, n2, d, g1.ptr, g2.ptr, C.byref(p), out.ptr))
+    return _ret(out, store_result_in_device)
+
+
+def GenLargeKNNGraph(ctx: Context, vectors, num_shards: int, params: HdbParams | None = None,
+                     store_result_in_device: bool = False):
+    """GenLargeKNNGraph (gpuknn/gen_large_knngraph.cu:409-484) on one device,
+    in memory: per-block NN-Descent + all-pairs knnmerge + unique merge."""
+    X, n, d = _as_points(ctx, vectors)
+    p = params or default_params()
+    out = ctx.empty((n, K), NND_ELEMENT)
+    ctx._check(ctx.lib.hdb_build_knng_blocked(ctx.handle, X.ptr, n, d, int(num_shards), C.byref(p), out.ptr))
+    return _ret(out, store_result_in_device)
+
+
+def bruteforce_knn(ctx: Context, vectors, query_ids, k: int = 32):
+    """Exact FP32 kNN of the given query ids (sampled recall check)."""
+    X, n, d = _as_points(ctx, vectors)
+    q = ctx.dev(np.ascontiguousarray(query_ids, np.int32))
+    m = q.shape[0]
+    ids = ctx.empty((m, k), np.int32)
+    d2 = ctx.empty((m, k), np.float32)
+    ctx._check(ctx.lib.hdb_bruteforce_knn(ctx.handle, X.ptr, n, d, q.ptr, m, int(k), ids.ptr, d2.ptr))
+    return ids.numpy(), d2.numpy()
+
+
+# ------------------------------------------------------ HDBSCAN stage --
+def calculate_degrees(ctx: Context, knn_lists, store_result_in_device: bool = False):
+    """calculate_degrees (HDBSCAN/calculates/calculates.cu:448-473): in-degree counts."""
+    L = ctx.dev(knn_lists, NND_ELEMENT)
+    n = L.shape[0]
+    counts = ctx.empty((n,), np.int32)
+    ctx._check(ctx.lib.hdb_hub_counts(ctx.handle, L.ptr, n, K, counts.ptr))
+    return _ret(counts, store_result_in_device)
+
+
+def find_antihubs(ctx: Context, knn_lists, counts):
+    """Anti-hub selection of buildEnhancedKNNG (graph.cu:108-151): sorted ids."""
+    L = ctx.dev(knn_lists, NND_ELEMENT)
+    c = ctx.dev(counts, np.int32)
+    n = L.shape[0]
+    hmax = int(np.sqrt(n)) + 2
+    ids = ctx.empty((hmax,), np.int32)
+    h = C.c_int64()
+    ctx._check(ctx.lib.hdb_antihubs(ctx.handle, L.ptr, c.ptr, n, K, ids.ptr, C.byref(h)))
+    return ids.numpy()[: h.value]
+
+
+def calculateCoreDistance(ctx: Context, vectors, knn_lists, mpts: int = 32, store_result_in_device=False):
+    """calculateCoreDistance (calculates.cu:185-197): Euclidean distance to entry mpts-1."""
+    X, n, d = _as_points(ctx, vectors)
+    L = ctx.dev(knn_lists, NND_ELEMENT)
+    core = ctx.empty((n,), np.float32)
+    ctx._check(ctx.lib.hdb_core_distances(ctx.handle, X.ptr, L.ptr, n, d, K, int(mpts), core.ptr))
+    return _ret(core, store_result_in_device)
+
+
+@dataclass
+class ECLgraph:
+    """ECLgraph (HDBSCAN/structs/ECLgraph.h:41-47) on the host."""
+    nodes: int
+    edges: int
+    nindex: np.ndarray
+    nlist: np.ndarray
+    eweight: np.ndarray
+
+
+def buildEnhancedKNNG(ctx: Context, vectors, knn_lists, mpts: int = 32) -> ECLgraph:
+    """buildEnhancedKNNG (graph.cu:108-176): hubs -> anti-hubs -> CSR -> core -> MRD."""
+    X, n, d = _as_points(ctx, vectors)
+    L = ctx.dev(knn_lists, NND_ELEMENT)
+    counts = calculate_degrees(ctx, L, store_result_in_device=True)
+    ah = find_antihubs(ctx, L, counts)
+    ahd = ctx.to_device(ah.astype(np.int32)) if len(ah) else ctx.empty((1,), np.int32)
+    core = calculateCoreDistance(ctx, X, L, mpts, store_result_in_device=True)
+    nindex = ctx.empty((n + 1,), np.int64)
+    E = C.c_int64()
+    ctx._check(ctx.lib.hdb_mrd_graph(ctx.handle, X.ptr, L.ptr, core.ptr, ahd.ptr, len(ah), n, d, K, nindex.ptr,
+                                     None, None, C.byref(E)))
+    nlist = ctx.empty((max(E.value, 1),), np.int32)
+    w = ctx.empty((max(E.value, 1),), np.float32)
+    ctx._check(ctx.lib.hdb_mrd_graph(ctx.handle, X.ptr, L.ptr, core.ptr, ahd.ptr, len(ah), n, d, K, nindex.ptr,
+                                     nlist.ptr, w.ptr, C.byref(E)))
+    return ECLgraph(n, E.value, nindex.numpy(), nlist.numpy()[: E.value], w.numpy()[: E.value])
+
+
+def gpuMST(ctx: Context, g: ECLgraph):
+    """gpuMST + buildMST (mst.cu:210-334): MST edges in CSR order, plus the
+    number of components (the reference exits when it is not 1)."""
+    n = g.nodes
+    ni = ctx.to_device(np.ascontiguousarray(g.nindex, np.int64))
+    nl = ctx.to_device(np.ascontiguousarray(g.nlist, np.int32)) if g.edges else ctx.empty((1,), np.int32)
+    w = ctx.to_device(np.ascontiguousarray(g.eweight, np.float32)) if g.edges else ctx.empty((1,), np.float32)
+    out = ctx.empty((max(n - 1, 1),), MST_EDGE)
+    m, comps = C.c_int64(), C.c_int64()
+    ctx._check(ctx.lib.hdb_mst(ctx.handle, ni.ptr, nl.ptr, w.ptr, n, out.ptr, C.byref(m), C.byref(comps)),
+               allow=(_lib.HDB_EDISCONNECTED,))
+    return out.numpy()[: m.value], comps.value
+
+
+def sort_edges(ctx: Context, edges):
+    """sort_edges (merge_sort.cu:226-279): stable ascending sort by weight."""
+    e = np.ascontiguousarray(edges, MST_EDGE)
+    if len(e) <= 1:
+        return e.copy()
+    de = ctx.to_device(e)
+    ctx._check(ctx.lib.hdb_sort_edges(ctx.handle, de.ptr, len(e)))
+    return de.numpy()
+
+
+def get_clusters(edges_sorted, n: int, mpts: int = 32):
+    """build_Linkage_tree .. get_clusters (tree.cu:171-746) on the host:
+    returns (labels, number of clusters)."""
+    lib = _lib.load()
+    e = np.ascontiguousarray(edges_sorted, MST_EDGE)
+    labels = np.empty(n, np.int32)
+    nc = C.c_int32()
+    st = lib.hdb_labels(e.ctypes.data if len(e) else None, int(n), int(mpts), labels.ctypes.data, C.byref(nc))
+    if st != _lib.HDB_OK:
+        raise HdbError(st, "hdb_labels failed")
+    return labels, nc.value
+
+
+@dataclass
+class PipelineResult:
+    lists: object
+    counts: object
+    core: object
+    edges: object
+    n_edges: int
+    components: int
+    csr_edges: int
+
+
+def pipeline(ctx: Context, vectors, params: HdbParams | None = None, knng_mode: int = 0, blocks: int = 1,
+             keep_on_device: bool = False, outputs: dict | None = None) -> PipelineResult:
+    """kNNG -> hub counts -> anti-hubs -> core -> MRD -> MST -> sorted edges
+    (the call order of HDBSCAN/_new.cu:149-314)."""
+    X, n, d = _as_points(ctx, vectors)
+    p = params or default_params()
+    o = HdbPipelineOpts()
+    o.knng_mode = int(knng_mode)
+    o.blocks = int(blocks)
+    bufs = outputs or {}
+    lists = bufs.get("lists") or ctx.empty((n, K), NND_ELEMENT)
+    counts = bufs.get("counts") or ctx.empty((n,), np.int32)
+    core = bufs.get("core") or ctx.empty((n,), np.float32)
+    edges = bufs.get("edges") or ctx.empty((max(n - 1, 1),), MST_EDGE)
+    m, comps, E = C.c_int64(), C.c_int64(), C.c_int64()
+    ctx._check(ctx.lib.hdb_pipeline(ctx.handle, X.ptr, n, d, C.byref(p), C.byref(o), lists.ptr, counts.ptr, core.ptr,
+                                    edges.ptr, C.byref(m), C.byref(comps), C.byref(E)),
+               allow=(_lib.HDB_EDISCONNECTED,))
+    if keep_on_device:
+        return PipelineResult(lists, counts, core, edges, m.value, comps.value, E.value)
+    return PipelineResult(lists.numpy(), counts.numpy(), core.numpy(), edges.numpy()[: m.value], m.value,
+                          comps.value, E.value)

@@ -1,0 +1,391 @@
+// extern "C" entry points of libhdbscan_mi355x.so: context, scratch arena,
+// stage timer and the hot-path functions declared in include/hdbscan_mi355x.h.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "hdb_internal.hpp"
+
+namespace hdb {
+
+int hip_fail(hdb_ctx* ctx, hipError_t e, const char* what, const char* file, int line) {
+  char buf[512];
+  std::snprintf(buf, sizeof(buf), "%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+  ctx->err = buf;
+  return e == hipErrorOutOfMemory ? HDB_ENOMEM : HDB_EDEVICE;
+}
+
+int scratch(hdb_ctx* ctx, Slot slot, size_t bytes, void** out) {
+  if (bytes == 0) bytes = 256;
+  if (ctx->slot_bytes[slot] < bytes) {
+    if (ctx->slot_ptr[slot]) {
+      HDB_HIP(hipStreamSynchronize(ctx->stream));
+      HDB_HIP(hipFree(ctx->slot_ptr[slot]));
+      ctx->slot_ptr[slot] = nullptr;
+      ctx->slot_bytes[slot] = 0;
+    }
+    const size_t want = bytes + bytes / 8 + 256;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      char buf[160];
+      std::snprintf(buf, sizeof(buf), "scratch allocation of %zu bytes failed", want);
+      ctx->err = buf;
+      return HDB_ENOMEM;
+    }
+    ctx->slot_ptr[slot] = p;
+    ctx->slot_bytes[slot] = want;
+  }
+  *out = ctx->slot_ptr[slot];
+  return HDB_OK;
+}
+
+Stage::Stage(hdb_ctx* c, const char* name) : ctx(c), on(c->timing) {
+  if (!on) return;
+  ev.name = name;
+  if (hipEventCreate(&ev.a) != hipSuccess || hipEventCreate(&ev.b) != hipSuccess) {
+    on = false;
+    return;
+  }
+  (void)hipEventRecord(ev.a, ctx->stream);
+}
+
+Stage::~Stage() {
+  if (!on) return;
+  (void)hipEventRecord(ev.b, ctx->stream);
+  ctx->pending.push_back(ev);
+}
+
+static int check_ctx(hdb_ctx* ctx) { return ctx ? HDB_OK : HDB_EINVAL; }
+
+static int set_device(hdb_ctx* ctx) {
+  HDB_HIP(hipSetDevice(ctx->device));
+  return HDB_OK;
+}
+
+}  // namespace hdb
+
+using namespace hdb;
+
+extern "C" {
+
+void hdb_params_default(hdb_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->k = 32;
+  p->sample_num = 16;
+  p->nnd_iteration = 6;
+  p->merge_sample_num = 12;
+  p->merge_iteration = 11;
+  p->mpts = 32;
+  p->distance_mode = 0;
+  p->rerank_keep = 48;
+  p->bucket_cap = 96;
+}
+
+int hdb_ctx_create(hdb_ctx** out, int device) {
+  if (!out) return HDB_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return HDB_EDEVICE;
+  if (device < 0 || device >= ndev) return HDB_EINVAL;
+  hdb_ctx* ctx = new (std::nothrow) hdb_ctx();
+  if (!ctx) return HDB_ENOMEM;
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return HDB_EDEVICE;
+  }
+  *out = ctx;
+  return HDB_OK;
+}
+
+int hdb_ctx_destroy(hdb_ctx* ctx) {
+  if (!ctx) return HDB_EINVAL;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& p : ctx->pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (int s = 0; s < kSlotCount; ++s)
+    if (ctx->slot_ptr[s]) (void)hipFree(ctx->slot_ptr[s]);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return HDB_OK;
+}
+
+const char* hdb_last_error(const hdb_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int hdb_synchronize(hdb_ctx* ctx) {
+  HDB_CALL(check_ctx(ctx));
+  HDB_HIP(hipStreamSynchronize(ctx->stream));
+  return HDB_OK;
+}
+
+int hdb_dev_alloc(hdb_ctx* ctx, void** ptr, size_t bytes) {
+  HDB_CALL(check_ctx(ctx));
+  if (!ptr) return ctx->fail(HDB_EINVAL, "null output pointer");
+  HDB_CALL(set_device(ctx));
+  *ptr = nullptr;
+  hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return ctx->fail(HDB_ENOMEM, "hipMalloc failed");
+  }
+  return HDB_OK;
+}
+
+int hdb_dev_free(hdb_ctx* ctx, void* ptr) {
+  HDB_CALL(check_ctx(ctx));
+  if (!ptr) return HDB_OK;
+  HDB_HIP(hipStreamSynchronize(ctx->stream));
+  HDB_HIP(hipFree(ptr));
+  return HDB_OK;
+}
+
+int hdb_memcpy_h2d(hdb_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  HDB_CALL(check_ctx(ctx));
+  if (bytes == 0) return HDB_OK;
+  if (!dst || !src) return ctx->fail(HDB_EINVAL, "null pointer");
+  HDB_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  HDB_HIP(hipStreamSynchronize(ctx->stream));
+  return HDB_OK;
+}
+
+int hdb_memcpy_d2h(hdb_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  HDB_CALL(check_ctx(ctx));
+  if (bytes == 0) return HDB_OK;
+  if (!dst || !src) return ctx->fail(HDB_EINVAL, "null pointer");
+  HDB_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HDB_HIP(hipStreamSynchronize(ctx->stream));
+  return HDB_OK;
+}
+
+int hdb_memcpy_d2d(hdb_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  HDB_CALL(check_ctx(ctx));
+  if (bytes == 0) return HDB_OK;
+  if (!dst || !src) return ctx->fail(HDB_EINVAL, "null pointer");
+  HDB_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  return HDB_OK;
+}
+
+int hdb_timer_enable(hdb_ctx* ctx, int enable) {
+  HDB_CALL(check_ctx(ctx));
+  ctx->timing = enable != 0;
+  return HDB_OK;
+}
+
+int hdb_timer_reset(hdb_ctx* ctx) {
+  HDB_CALL(check_ctx(ctx));
+  HDB_HIP(hipStreamSynchronize(ctx->stream));
+  for (auto& p : ctx->pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  ctx->pending.clear();
+  ctx->timers.clear();
+  return HDB_OK;
+}
+
+int hdb_timer_read(hdb_ctx* ctx, const char** names, double* ms, int64_t* launches, int max_entries,
+                   int* n_entries) {
+  HDB_CALL(check_ctx(ctx));
+  HDB_HIP(hipStreamSynchronize(ctx->stream));
+  for (auto& p : ctx->pending) {
+    float t = 0.0f;
+    if (hipEventElapsedTime(&t, p.a, p.b) == hipSuccess) {
+      TimerAcc& acc = ctx->timers[p.name];
+      acc.ms += t;
+      acc.launches += 1;
+    }
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  ctx->pending.clear();
+  ctx->timer_names.clear();
+  for (auto& kv : ctx->timers) ctx->timer_names.push_back(kv.first);
+  int i = 0;
+  for (auto& kv : ctx->timers) {
+    if (i >= max_entries) break;
+    if (names) names[i] = ctx->timer_names[i].c_str();
+    if (ms) ms[i] = kv.second.ms;
+    if (launches) launches[i] = kv.second.launches;
+    ++i;
+  }
+  if (n_entries) *n_entries = static_cast<int>(ctx->timers.size());
+  return HDB_OK;
+}
+
+int hdb_nnd_stats(hdb_ctx* ctx, int64_t* proposals, int64_t* bucket_drops, int64_t* reverse_drops) {
+  HDB_CALL(check_ctx(ctx));
+  if (proposals) *proposals = ctx->stats_host[0];
+  if (bucket_drops) *bucket_drops = ctx->stats_host[1];
+  if (reverse_drops) *reverse_drops = ctx->stats_host[2];
+  return HDB_OK;
+}
+
+int hdb_nndescent(hdb_ctx* ctx, const float* X, int64_t n, int32_t d, const hdb_params* params,
+                  hdb_nnd_element* out) {
+  HDB_CALL(check_ctx(ctx));
+  if (!params) return ctx->fail(HDB_EINVAL, "null params");
+  HDB_CALL(set_device(ctx));
+  return nnd_run(ctx, X, n, d, *params, out, true, params->nnd_iteration);
+}
+
+int hdb_nndescent_from(hdb_ctx* ctx, const float* X, int64_t n, int32_t d, const hdb_params* params,
+                       int32_t iterations, hdb_nnd_element* lists) {
+  HDB_CALL(check_ctx(ctx));
+  if (!params || iterations < 0) return ctx->fail(HDB_EINVAL, "bad arguments");
+  HDB_CALL(set_device(ctx));
+  return nnd_run(ctx, X, n, d, *params, lists, false, iterations);
+}
+
+int hdb_knn_merge(hdb_ctx* ctx, const float* X, int64_t n1, int64_t n2, int32_t d, const hdb_nnd_element* g1,
+                  const hdb_nnd_element* g2, const hdb_params* params, hdb_nnd_element* out) {
+  HDB_CALL(check_ctx(ctx));
+  if (!params) return ctx->fail(HDB_EINVAL, "null params");
+  HDB_CALL(set_device(ctx));
+  return nnd_merge_pair(ctx, X, n1, n2, d, g1, g2, *params, out);
+}
+
+int hdb_build_knng_blocked(hdb_ctx* ctx, const float* X, int64_t n, int32_t d, int32_t blocks,
+                           const hdb_params* params, hdb_nnd_element* out) {
+  HDB_CALL(check_ctx(ctx));
+  if (!params || !X || !out) return ctx->fail(HDB_EINVAL, "null argument");
+  HDB_CALL(set_device(ctx));
+  return nnd_blocked(ctx, X, n, d, blocks, *params, out);
+}
+
+int hdb_bruteforce_knn(hdb_ctx* ctx, const float* X, int64_t n, int32_t d, const int32_t* qids, int64_t m,
+                       int32_t kq, int32_t* out_ids, float* out_d2) {
+  HDB_CALL(check_ctx(ctx));
+  HDB_CALL(set_device(ctx));
+  return bruteforce_knn(ctx, X, n, d, qids, m, kq, out_ids, out_d2);
+}
+
+int hdb_hub_counts(hdb_ctx* ctx, const hdb_nnd_element* lists, int64_t n, int32_t k, int32_t* counts) {
+  HDB_CALL(check_ctx(ctx));
+  HDB_CALL(set_device(ctx));
+  return hub_counts(ctx, lists, n, k, counts);
+}
+
+int hdb_antihubs(hdb_ctx* ctx, const hdb_nnd_element* lists, const int32_t* counts, int64_t n, int32_t k,
+                 int32_t* ids, int64_t* h_out) {
+  HDB_CALL(check_ctx(ctx));
+  HDB_CALL(set_device(ctx));
+  return antihubs(ctx, lists, counts, n, k, ids, h_out);
+}
+
+int hdb_core_distances(hdb_ctx* ctx, const float* X, const hdb_nnd_element* lists, int64_t n, int32_t d,
+                       int32_t k, int32_t mpts, float* core) {
+  HDB_CALL(check_ctx(ctx));
+  HDB_CALL(set_device(ctx));
+  return core_distances(ctx, X, lists, n, d, k, mpts, core);
+}
+
+int hdb_mrd_graph(hdb_ctx* ctx, const float* X, const hdb_nnd_element* lists, const float* core,
+                  const int32_t* ah, int64_t h, int64_t n, int32_t d, int32_t k, int64_t* nindex, int32_t* nlist,
+                  float* w, int64_t* edges_out) {
+  HDB_CALL(check_ctx(ctx));
+  HDB_CALL(set_device(ctx));
+  return mrd_graph(ctx, X, lists, core, ah, h, n, d, k, false, nindex, nlist, w, nullptr, edges_out);
+}
+
+int hdb_mst(hdb_ctx* ctx, const int64_t* nindex, const int32_t* nlist, const float* w, int64_t n,
+            hdb_mst_edge* out, int64_t* m_out, int64_t* components_out) {
+  HDB_CALL(check_ctx(ctx));
+  if (!nindex || !nlist || !w || !out || !m_out || !components_out || n < 1)
+    return ctx->fail(HDB_EINVAL, "bad hdb_mst arguments");
+  HDB_CALL(set_device(ctx));
+  int64_t E = 0;
+  HDB_CALL(read_i64(ctx, nindex + n, &E));
+  int64_t m = 0;
+  HDB_CALL(csr_upper(ctx, nindex, nlist, w, n, E, nullptr, nullptr, nullptr, &m));
+  int32_t *usrc, *udst;
+  float* uw;
+  uint8_t* flags;
+  HDB_CALL(scratch_t(ctx, kSlotPipeSrc, std::max<int64_t>(m, 1), &usrc));
+  HDB_CALL(scratch_t(ctx, kSlotPipeNlist, std::max<int64_t>(m, 1), &udst));
+  HDB_CALL(scratch_t(ctx, kSlotPipeW, std::max<int64_t>(m, 1), &uw));
+  HDB_CALL(scratch_t(ctx, kSlotMstFlag, std::max<int64_t>(m, 1), &flags));
+  HDB_CALL(csr_upper(ctx, nindex, nlist, w, n, E, usrc, udst, uw, &m));
+  HDB_CALL(mst_edges_boruvka(ctx, usrc, udst, uw, n, m, flags, components_out));
+  HDB_CALL(mst_extract(ctx, usrc, udst, uw, flags, m, out, std::max<int64_t>(n - 1, 0), m_out));
+  return *components_out > 1 ? ctx->fail(HDB_EDISCONNECTED, "graph is disconnected: spanning forest returned")
+                             : HDB_OK;
+}
+
+int hdb_sort_edges(hdb_ctx* ctx, hdb_mst_edge* edges, int64_t m) {
+  HDB_CALL(check_ctx(ctx));
+  if (m > 0 && !edges) return ctx->fail(HDB_EINVAL, "null edges");
+  HDB_CALL(set_device(ctx));
+  return sort_edges(ctx, edges, m);
+}
+
+int hdb_pipeline(hdb_ctx* ctx, const float* X, int64_t n, int32_t d, const hdb_params* params,
+                 const hdb_pipeline_opts* opts, hdb_nnd_element* lists, int32_t* counts, float* core,
+                 hdb_mst_edge* edges, int64_t* m_out, int64_t* components_out, int64_t* csr_edges_out) {
+  HDB_CALL(check_ctx(ctx));
+  if (!params || !X || !lists || !counts || !core || !edges || !m_out || !components_out)
+    return ctx->fail(HDB_EINVAL, "null argument");
+  if (params->mpts < 1 || params->mpts > params->k) return ctx->fail(HDB_EINVAL, "mpts must be in [1, k]");
+  HDB_CALL(set_device(ctx));
+  hdb_pipeline_opts o;
+  std::memset(&o, 0, sizeof(o));
+  if (opts) o = *opts;
+  Stage total(ctx, "pipeline_total");
+  // kNN graph (ConstructLargeKNNGraph -> GenLargeKNNGraph, _new.cu:190)
+  {
+    Stage st(ctx, "stage_knng");
+    if (o.knng_mode == 1) HDB_CALL(nnd_blocked(ctx, X, n, d, o.blocks, *params, lists));
+    else HDB_CALL(nnd_run(ctx, X, n, d, *params, lists, true, params->nnd_iteration));
+  }
+  // buildEnhancedKNNG (graph.cu:108-176)
+  const int64_t hmax = static_cast<int64_t>(std::sqrt(static_cast<double>(n))) + 2;
+  int32_t* ah;
+  HDB_CALL(scratch_t(ctx, kSlotPipeAh, hmax, &ah));
+  int64_t h = 0;
+  {
+    Stage st(ctx, "stage_hubs_antihubs");
+    HDB_CALL(hub_counts(ctx, lists, n, params->k, counts));
+    HDB_CALL(antihubs(ctx, lists, counts, n, params->k, ah, &h));
+  }
+  {
+    Stage st(ctx, "stage_core");
+    HDB_CALL(core_distances(ctx, X, lists, n, d, params->k, params->mpts, core));
+  }
+  const int64_t cap = n * params->k + h * (h - 1) / 2 + 1;
+  int64_t* nindex;
+  int32_t *nlist, *src;
+  float* w;
+  HDB_CALL(scratch_t(ctx, kSlotPipeNindex, n + 1, &nindex));
+  HDB_CALL(scratch_t(ctx, kSlotPipeNlist, cap, &nlist));
+  HDB_CALL(scratch_t(ctx, kSlotPipeSrc, cap, &src));
+  HDB_CALL(scratch_t(ctx, kSlotPipeW, cap, &w));
+  int64_t m = 0;
+  {
+    Stage st(ctx, "stage_mrd_graph");
+    HDB_CALL(mrd_graph(ctx, X, lists, core, ah, h, n, d, params->k, true, nindex, nlist, w, src, &m));
+  }
+  ctx->csr_edges = 2 * m;
+  if (csr_edges_out) *csr_edges_out = 2 * m;
+  // cpuMST/gpuMST + buildMST (_new.cu:298-314)
+  uint8_t* flags;
+  HDB_CALL(scratch_t(ctx, kSlotMstFlag, std::max<int64_t>(m, 1), &flags));
+  {
+    Stage st(ctx, "stage_mst");
+    HDB_CALL(mst_edges_boruvka(ctx, src, nlist, w, n, m, flags, components_out));
+    HDB_CALL(mst_extract(ctx, src, nlist, w, flags, m, edges, std::max<int64_t>(n - 1, 0), m_out));
+  }
+  {
+    Stage st(ctx, "stage_sort");
+    HDB_CALL(sort_edges(ctx, edges, *m_out));
+  }
+  return *components_out > 1 ? ctx->fail(HDB_EDISCONNECTED, "graph is disconnected: spanning forest returned")
+                             : HDB_OK;
+}
+
+}  // extern "C"

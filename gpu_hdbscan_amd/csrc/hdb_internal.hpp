@@ -1,0 +1,304 @@
+// Internal declarations shared by the gfx950 translation units of
+// libhdbscan_mi355x.so: the context, scratch arena, stage timer, error
+// plumbing, wave64 helpers and the launch wrappers of each kernel family.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/hdbscan_mi355x.h"
+
+namespace hdb {
+
+constexpr int kK = 32;                       // NEIGHB_NUM_PER_LIST
+constexpr uint32_t kInvalidId = 0x7FFFFFFFu;  // empty list slot label
+constexpr uint64_t kKeyMax = ~0ull;
+constexpr int kMaxBucketCap = 96;             // 128-wide wave sort minus the list
+constexpr int kWave = 64;
+
+// Scratch arena slots owned by a context (grown on demand, never shrunk).
+enum Slot : int {
+  kSlotFwdNew, kSlotFwdOld, kSlotRevNew, kSlotRevOld, kSlotCounts4,  // sampling
+  kSlotBucket, kSlotBucketCnt, kSlotWorst, kSlotStats,                // list update
+  kSlotBlockLocal, kSlotPairGraph, kSlotPairX, kSlotPairG1, kSlotPairG2,  // blocked build
+  kSlotScanTmp, kSlotRowCnt, kSlotRowCursor, kSlotRowTmp, kSlotRowUniq,  // CSR build
+  kSlotAhFlag, kSlotAhAux, kSlotAhKeys, kSlotAhKeys2,                 // anti-hubs
+  kSlotMstParent, kSlotMstMin, kSlotMstWl1, kSlotMstWl2, kSlotMstFlag, kSlotMstCnt,
+  kSlotMstSrc, kSlotMstPos,
+  kSlotSortKeys, kSlotSortVals, kSlotSortHist, kSlotSortAux,          // radix sort
+  kSlotPipeNindex, kSlotPipeNlist, kSlotPipeW, kSlotPipeAh, kSlotPipeSrc,  // pipeline
+  kSlotBfTmp, kSlotMisc0, kSlotMisc1, kSlotMisc2,
+  kSlotCount
+};
+
+struct TimerAcc {
+  double ms = 0.0;
+  int64_t launches = 0;
+};
+
+struct PendingEvent {
+  std::string name;
+  hipEvent_t a, b;
+};
+
+}  // namespace hdb
+
+struct hdb_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  void* slot_ptr[hdb::kSlotCount] = {};
+  size_t slot_bytes[hdb::kSlotCount] = {};
+  bool timing = false;
+  std::vector<hdb::PendingEvent> pending;
+  std::map<std::string, hdb::TimerAcc> timers;
+  std::vector<std::string> timer_names;  // stable storage for hdb_timer_read
+  int64_t stats_host[4] = {0, 0, 0, 0};
+  // Pipeline CSR (kept when requested, for tests).
+  int64_t csr_edges = 0;
+
+  int fail(int code, const std::string& msg) {
+    err = msg;
+    return code;
+  }
+};
+
+namespace hdb {
+
+int hip_fail(hdb_ctx* ctx, hipError_t e, const char* what, const char* file, int line);
+
+#define HDB_HIP(expr)                                                    \
+  do {                                                                   \
+    hipError_t e_ = (expr);                                              \
+    if (e_ != hipSuccess) return hdb::hip_fail(ctx, e_, #expr, __FILE__, __LINE__); \
+  } while (0)
+
+#define HDB_LAUNCHED()                                                   \
+  do {                                                                   \
+    hipError_t e_ = hipGetLastError();                                   \
+    if (e_ != hipSuccess) return hdb::hip_fail(ctx, e_, "kernel launch", __FILE__, __LINE__); \
+  } while (0)
+
+#define HDB_CALL(expr)             \
+  do {                             \
+    int s_ = (expr);               \
+    if (s_ != HDB_OK) return s_;   \
+  } while (0)
+
+// Scratch buffer of at least `bytes` for `slot`; contents are undefined.
+int scratch(hdb_ctx* ctx, Slot slot, size_t bytes, void** out);
+template <typename T>
+inline int scratch_t(hdb_ctx* ctx, Slot slot, size_t count, T** out) {
+  return scratch(ctx, slot, count * sizeof(T), reinterpret_cast<void**>(out));
+}
+
+// Stage timer: records a HIP event pair around the enclosed launches.
+struct Stage {
+  hdb_ctx* ctx;
+  PendingEvent ev;
+  bool on;
+  Stage(hdb_ctx* c, const char* name);
+  ~Stage();
+};
+
+inline unsigned grid_for(int64_t threads, int block) {
+  return static_cast<unsigned>((threads + block - 1) / block);
+}
+
+// ---- device helpers ------------------------------------------------------
+
+// xorshift64* as used by the reference init (gpuknn/knncuda_tools.cu:126-132):
+// Vigna's xorshift64* step applied to (x + 42).
+__host__ __device__ inline uint64_t xorshift64star(uint64_t x) {
+  x += 42ull;
+  x ^= x >> 12;
+  x ^= x << 25;
+  x ^= x >> 27;
+  return x * 0x2545F4914F6CDD1Dull;
+}
+
+// Squared L2 distance in FP32, accumulated in dimension order with fused
+// multiply-add.  Every distance of the kNN path goes through this one
+// formula, so the same pair always yields the same bits (the oracle uses the
+// identical formula with C fmaf).
+template <int D>
+__device__ __forceinline__ float d2_fixed(const float* __restrict__ a, const float* __restrict__ b) {
+  float acc = 0.0f;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    float diff = a[k] - b[k];
+    acc = __builtin_fmaf(diff, diff, acc);
+  }
+  return acc;
+}
+
+__device__ __forceinline__ float d2_any(const float* __restrict__ a, const float* __restrict__ b, int d) {
+  float acc = 0.0f;
+  for (int k = 0; k < d; ++k) {
+    float diff = a[k] - b[k];
+    acc = __builtin_fmaf(diff, diff, acc);
+  }
+  return acc;
+}
+
+// Euclidean distance accumulated in FP64 (no contraction), rounded once.
+__device__ __forceinline__ float dist_fp64(const float* __restrict__ a, const float* __restrict__ b, int d) {
+  double acc = 0.0;
+  for (int k = 0; k < d; ++k) {
+    double diff = __dsub_rn(static_cast<double>(a[k]), static_cast<double>(b[k]));
+    acc = __dadd_rn(acc, __dmul_rn(diff, diff));
+  }
+  return static_cast<float>(__dsqrt_rn(acc));
+}
+
+// List key: (squared distance bits, id, new flag).  Non-negative floats order
+// like their bit patterns, so an unsigned compare orders by distance, then
+// id, then old-before-new.
+__device__ __forceinline__ uint64_t make_key(float d2, uint32_t id, uint32_t is_new) {
+  return (static_cast<uint64_t>(__float_as_uint(d2)) << 32) | (static_cast<uint64_t>(id) << 1) | is_new;
+}
+__device__ __forceinline__ float key_d2(uint64_t k) { return __uint_as_float(static_cast<uint32_t>(k >> 32)); }
+__device__ __forceinline__ uint32_t key_id(uint64_t k) { return static_cast<uint32_t>(k) >> 1; }
+__device__ __forceinline__ uint32_t key_new(uint64_t k) { return static_cast<uint32_t>(k) & 1u; }
+
+__device__ __forceinline__ uint32_t label_id(int32_t label) {
+  return label >= 0 ? static_cast<uint32_t>(label) : static_cast<uint32_t>(-label - 1);
+}
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int mask) {
+  uint32_t lo = static_cast<uint32_t>(v), hi = static_cast<uint32_t>(v >> 32);
+  lo = __shfl_xor(lo, mask);
+  hi = __shfl_xor(hi, mask);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
+  uint32_t lo = static_cast<uint32_t>(v), hi = static_cast<uint32_t>(v >> 32);
+  lo = __shfl(lo, src);
+  hi = __shfl(hi, src);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const unsigned lane = __lane_id();
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Bitonic sort (ascending) of one u64 per lane within aligned groups of
+// GROUP lanes (GROUP = 32 or 64).
+template <int GROUP>
+__device__ __forceinline__ uint64_t bitonic_group(uint64_t v) {
+  const unsigned lane = __lane_id() & (GROUP - 1);
+#pragma unroll
+  for (int k = 2; k <= GROUP; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      uint64_t o = shfl_xor_u64(v, j);
+      bool up = (k == GROUP) ? true : ((lane & k) == 0);
+      bool low = (lane & j) == 0;
+      uint64_t mn = v < o ? v : o, mx = v < o ? o : v;
+      v = (low == up) ? mn : mx;
+    }
+  }
+  return v;
+}
+template <int GROUP>
+__device__ __forceinline__ uint32_t bitonic_group_u32(uint32_t v) {
+  const unsigned lane = __lane_id() & (GROUP - 1);
+#pragma unroll
+  for (int k = 2; k <= GROUP; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      uint32_t o = __shfl_xor(v, j);
+      bool up = (k == GROUP) ? true : ((lane & k) == 0);
+      bool low = (lane & j) == 0;
+      uint32_t mn = v < o ? v : o, mx = v < o ? o : v;
+      v = (low == up) ? mn : mx;
+    }
+  }
+  return v;
+}
+
+// Bitonic sort (ascending) of 128 u64 keys held as 2 per lane: element
+// index i = r * 64 + lane for register r.
+__device__ __forceinline__ void bitonic128(uint64_t& r0, uint64_t& r1) {
+  const unsigned lane = __lane_id();
+#pragma unroll
+  for (int k = 2; k <= 128; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j == 64) {
+        // partner is the other register of the same lane; i = lane (< 64)
+        // and i ^ 64 = 64 + lane; k == 128 here, so ascending.
+        uint64_t mn = r0 < r1 ? r0 : r1, mx = r0 < r1 ? r1 : r0;
+        r0 = mn;
+        r1 = mx;
+      } else {
+        uint64_t o0 = shfl_xor_u64(r0, j), o1 = shfl_xor_u64(r1, j);
+        bool low = (lane & j) == 0;
+        // element indices lane (r0) and 64 + lane (r1)
+        bool up0 = (k == 128) ? true : ((lane & k) == 0);
+        bool up1 = (k == 128) ? true : (((64u + lane) & k) == 0);
+        uint64_t mn0 = r0 < o0 ? r0 : o0, mx0 = r0 < o0 ? o0 : r0;
+        uint64_t mn1 = r1 < o1 ? r1 : o1, mx1 = r1 < o1 ? o1 : r1;
+        r0 = (low == up0) ? mn0 : mx0;
+        r1 = (low == up1) ? mn1 : mx1;
+      }
+    }
+  }
+}
+
+// ---- kernel families (launch wrappers) -----------------------------------
+// prims.hip
+int fill_i32(hdb_ctx* ctx, int32_t* p, int64_t n, int32_t v);
+int fill_u64(hdb_ctx* ctx, uint64_t* p, int64_t n, uint64_t v);
+int fill_u8(hdb_ctx* ctx, uint8_t* p, int64_t n, uint8_t v);
+// Exclusive scan of n int32 counts into n+1 int64 offsets (out[n] = total).
+int scan_i32_to_i64(hdb_ctx* ctx, const int32_t* in, int64_t n, int64_t* out);
+// Stable LSD radix sort.  Keys u32 (bits [bit0, bit1)) with u64 payload.
+int radix_sort_u32_u64(hdb_ctx* ctx, uint32_t* keys, uint64_t* vals, int64_t n, int bit0, int bit1);
+// Keys-only u64 radix sort over bits [bit0, bit1).
+int radix_sort_u64(hdb_ctx* ctx, uint64_t* keys, int64_t n, int bit0, int bit1);
+int read_i64(hdb_ctx* ctx, const int64_t* p, int64_t* out);
+int read_i32(hdb_ctx* ctx, const int32_t* p, int32_t* out);
+
+// nndescent.hip
+int nnd_run(hdb_ctx* ctx, const float* X, int64_t n, int d, const hdb_params& prm,
+            hdb_nnd_element* lists, bool init_random, int iterations);
+int nnd_merge_pair(hdb_ctx* ctx, const float* X, int64_t n1, int64_t n2, int d,
+                   const hdb_nnd_element* g1, const hdb_nnd_element* g2, const hdb_params& prm,
+                   hdb_nnd_element* out);
+int nnd_blocked(hdb_ctx* ctx, const float* X, int64_t n, int d, int blocks, const hdb_params& prm,
+                hdb_nnd_element* out);
+int bruteforce_knn(hdb_ctx* ctx, const float* X, int64_t n, int d, const int32_t* qids, int64_t m,
+                   int kq, int32_t* out_ids, float* out_d2);
+
+// graph.hip
+int hub_counts(hdb_ctx* ctx, const hdb_nnd_element* lists, int64_t n, int k, int32_t* counts);
+int antihubs(hdb_ctx* ctx, const hdb_nnd_element* lists, const int32_t* counts, int64_t n, int k,
+             int32_t* ids, int64_t* h_out);
+int core_distances(hdb_ctx* ctx, const float* X, const hdb_nnd_element* lists, int64_t n, int d,
+                   int k, int mpts, float* core);
+// upper != 0: rows hold only neighbours with larger id (one entry per
+// undirected edge) and src receives each entry's row; upper == 0: full CSR.
+int mrd_graph(hdb_ctx* ctx, const float* X, const hdb_nnd_element* lists, const float* core,
+              const int32_t* ah, int64_t h, int64_t n, int d, int k, bool upper,
+              int64_t* nindex, int32_t* nlist, float* w, int32_t* src, int64_t* edges_out);
+
+// mst.hip
+// Minimum spanning forest of the undirected edge list (src[e] < dst[e], e in
+// canonical order) under the order (w, e).  flags[e] set for tree edges.
+int mst_edges_boruvka(hdb_ctx* ctx, const int32_t* src, const int32_t* dst, const float* w,
+                      int64_t n, int64_t m_edges, uint8_t* flags, int64_t* components_out);
+int mst_extract(hdb_ctx* ctx, const int32_t* src, const int32_t* dst, const float* w,
+                const uint8_t* flags, int64_t m_edges, hdb_mst_edge* out, int64_t cap, int64_t* count_out);
+int sort_edges(hdb_ctx* ctx, hdb_mst_edge* edges, int64_t m);
+// Upper-triangle source/position arrays of a full CSR (entry j of row i with
+// nlist[j] > i): writes src, dst, w of those entries in CSR order.
+int csr_upper(hdb_ctx* ctx, const int64_t* nindex, const int32_t* nlist, const float* w, int64_t n,
+              int64_t E, int32_t* usrc, int32_t* udst, float* uw, int64_t* m_out);
+
+}  // namespace hdb

@@ -1,0 +1,243 @@
+// Host labels: single linkage -> condensed tree -> stability -> EOM cluster
+// selection -> labels, with the semantics of HDBSCAN/trees/tree.cu:171-746
+// (including its quirks: lambda = (float)INT_MAX for zero-weight merges,
+// tree.cu:308-313, and a point labelled only when its lambda is strictly
+// above its cluster's birth lambda, tree.cu:609-618).  The reference's
+// quadratic scans (BFS_from_cluster_tree, getIndexes) are replaced by child
+// lists; the visit orders that decide floating-point sums are kept.
+//
+// Built with -ffp-contract=off: the stability sums must round exactly like
+// the reference's float arithmetic.
+#include <algorithm>
+#include <climits>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/hdbscan_mi355x.h"
+
+namespace {
+
+struct LinkNode {
+  int32_t left, right;
+  float weight;
+  int32_t size;
+};
+
+struct CondNode {
+  int32_t parent, child;
+  float lambda;
+  int32_t child_size;
+};
+
+// build_Linkage_tree (tree.cu:171-202) with UnionFind (tree.cu:9-87).
+std::vector<LinkNode> linkage(const hdb_mst_edge* e, int64_t n) {
+  std::vector<int32_t> parent(2 * n - 1, -1), size(2 * n - 1, 0);
+  for (int64_t i = 0; i < n; ++i) size[i] = 1;
+  int32_t next = static_cast<int32_t>(n);
+  auto find = [&](int32_t x) {
+    int32_t r = x;
+    while (parent[r] != -1) r = parent[r];
+    while (x != r) {  // full compression (roots are unaffected)
+      const int32_t nx = parent[x];
+      parent[x] = r;
+      x = nx;
+    }
+    return r;
+  };
+  std::vector<LinkNode> h(n - 1);
+  for (int64_t i = 0; i < n - 1; ++i) {
+    const int32_t aa = find(e[i].from_node), bb = find(e[i].to_node);
+    h[i] = {aa, bb, e[i].weight, size[aa] + size[bb]};
+    size[next] = size[aa] + size[bb];
+    parent[aa] = next;
+    parent[bb] = next;
+    ++next;
+  }
+  return h;
+}
+
+// BFS_from_hierarchy (tree.cu:206-253): level order, children left then right.
+void bfs_hierarchy(const std::vector<LinkNode>& h, int32_t root, int32_t npts, std::vector<int32_t>* out) {
+  out->clear();
+  std::vector<int32_t> level{root}, nxt;
+  while (!level.empty()) {
+    out->insert(out->end(), level.begin(), level.end());
+    nxt.clear();
+    for (int32_t x : level)
+      if (x >= npts) {
+        nxt.push_back(h[x - npts].left);
+        nxt.push_back(h[x - npts].right);
+      }
+    level.swap(nxt);
+  }
+}
+
+// build_Condensed_tree (tree.cu:255-442).
+std::vector<CondNode> condense(const std::vector<LinkNode>& h, int64_t n, int32_t mpts) {
+  const int32_t npts = static_cast<int32_t>(n);
+  const int32_t root = 2 * (npts - 1);
+  std::vector<int32_t> order, sub;
+  bfs_hierarchy(h, root, npts, &order);
+  std::vector<int32_t> relabel(root + 1, 0);
+  std::vector<uint8_t> ignore(root + 1, 0);
+  relabel[root] = npts;
+  int32_t nextlabel = npts + 1;
+  std::vector<CondNode> out;
+  out.reserve(static_cast<size_t>(n) + 64);
+  auto fall_out = [&](int32_t sub_root, int32_t parent_label, float lambda) {
+    bfs_hierarchy(h, sub_root, npts, &sub);
+    for (int32_t s : sub) {
+      if (s < npts) out.push_back({parent_label, s, lambda, 1});
+      ignore[s] = 1;
+    }
+  };
+  for (int32_t node : order) {
+    if (ignore[node] || node < npts) continue;
+    const LinkNode& x = h[node - npts];
+    float lambda = static_cast<float>(INT_MAX);
+    if (x.weight > 0.0) lambda = static_cast<float>(1.0 / x.weight);
+    const int32_t lc = x.left >= npts ? h[x.left - npts].size : 1;
+    const int32_t rc = x.right >= npts ? h[x.right - npts].size : 1;
+    if (lc >= mpts && rc >= mpts) {
+      relabel[x.left] = nextlabel++;
+      out.push_back({relabel[node], relabel[x.left], lambda, lc});
+      relabel[x.right] = nextlabel++;
+      out.push_back({relabel[node], relabel[x.right], lambda, rc});
+    } else if (lc < mpts && rc < mpts) {
+      fall_out(x.left, relabel[node], lambda);
+      fall_out(x.right, relabel[node], lambda);
+    } else if (lc < mpts) {
+      relabel[x.right] = relabel[node];
+      fall_out(x.left, relabel[node], lambda);
+    } else {
+      relabel[x.left] = relabel[node];
+      fall_out(x.right, relabel[node], lambda);
+    }
+  }
+  return out;
+}
+
+}  // namespace
+
+extern "C" int hdb_labels(const hdb_mst_edge* edges, int64_t n, int32_t mpts, int32_t* labels,
+                          int32_t* clusters_out) {
+  if (!labels || n < 1 || mpts < 2 || (n > 1 && !edges)) return HDB_EINVAL;
+  if (n >= (static_cast<int64_t>(1) << 30)) return HDB_EUNSUPPORTED;
+  if (clusters_out) *clusters_out = 0;
+  if (n == 1) {
+    labels[0] = -1;
+    return HDB_OK;
+  }
+  for (int64_t i = 0; i < n - 1; ++i) {
+    const int32_t a = edges[i].from_node, b = edges[i].to_node;
+    if (a < 0 || b < 0 || a >= n || b >= n) return HDB_EINVAL;
+  }
+  const std::vector<LinkNode> h = linkage(edges, n);
+  const std::vector<CondNode> ct = condense(h, n, mpts);
+  const int64_t csize = static_cast<int64_t>(ct.size());
+  if (csize == 0) {
+    for (int64_t i = 0; i < n; ++i) labels[i] = -1;
+    return HDB_OK;
+  }
+  // compute_stability (tree.cu:444-528)
+  int32_t max_child = ct[0].child, max_parent = ct[0].parent, min_parent = ct[0].parent;
+  for (const CondNode& c : ct) {
+    max_child = std::max(max_child, c.child);
+    max_parent = std::max(max_parent, c.parent);
+    min_parent = std::min(min_parent, c.parent);
+  }
+  const int32_t largest_child = std::max(max_child, min_parent);
+  std::vector<float> birth(largest_child + 1, 0.0f);
+  {
+    std::vector<uint8_t> seen(largest_child + 1, 0);
+    for (const CondNode& c : ct) {  // min lambda per child
+      if (!seen[c.child] || c.lambda < birth[c.child]) birth[c.child] = c.lambda;
+      seen[c.child] = 1;
+    }
+  }
+  birth[min_parent] = 0.0f;
+  const int32_t nclusters = max_parent - min_parent + 1;
+  std::vector<float> stab(nclusters, 0.0f);
+  for (const CondNode& c : ct) {
+    const float term = (c.lambda - birth[c.parent]) * static_cast<float>(c.child_size);
+    stab[c.parent - min_parent] += term;
+  }
+  // get_clusters (tree.cu:627-746): EOM, children before parents.
+  // cluster children in tree order: child_size > 1 for the subtree sums
+  // (cluster_tree, tree.cu:640-656); every cluster id for the descendant
+  // walk (BFS_from_cluster_tree, tree.cu:532-567)
+  std::vector<std::vector<int32_t>> kids(nclusters), allkids(nclusters);
+  for (const CondNode& c : ct) {
+    if (c.child_size > 1) kids[c.parent - min_parent].push_back(c.child);
+    if (c.child >= min_parent) allkids[c.parent - min_parent].push_back(c.child);
+  }
+  std::vector<uint8_t> is_cluster(nclusters, 1);
+  is_cluster[0] = 0;  // the root (i > min_parent only)
+  std::vector<int32_t> stack;
+  for (int32_t id = max_parent; id > min_parent; --id) {
+    const int32_t ix = id - min_parent;
+    float sub = 0.0f;
+    for (int32_t ch : kids[ix]) sub += stab[ch - min_parent];
+    if (sub > stab[ix]) {
+      is_cluster[ix] = 0;
+      stab[ix] = sub;
+    } else {
+      stack.assign(allkids[ix].begin(), allkids[ix].end());
+      while (!stack.empty()) {
+        const int32_t c = stack.back();
+        stack.pop_back();
+        is_cluster[c - min_parent] = 0;
+        for (int32_t g : allkids[c - min_parent]) stack.push_back(g);
+      }
+    }
+  }
+  std::vector<int32_t> cmap(nclusters, -1);
+  int32_t nsel = 0;
+  for (int32_t ix = 0; ix < nclusters; ++ix)
+    if (is_cluster[ix]) cmap[ix] = nsel++;
+  if (clusters_out) *clusters_out = nsel;
+  // do_labelling (tree.cu:569-625): lambda_array by child id, union-find of
+  // the non-selected edges.
+  std::vector<float> lam(csize + 1 > largest_child + 1 ? csize + 1 : largest_child + 1, 0.0f);
+  for (const CondNode& c : ct) {
+    lam[c.child] = c.lambda;
+    if (c.parent == min_parent) lam[min_parent] = c.lambda;
+  }
+  const int32_t usize = max_parent + 1;
+  std::vector<int32_t> up(usize), rank(usize, 0);
+  for (int32_t i = 0; i < usize; ++i) up[i] = i;
+  auto ufind = [&](int32_t x) {
+    int32_t r = x;
+    while (up[r] != r) r = up[r];
+    while (up[x] != r) {
+      const int32_t nx = up[x];
+      up[x] = r;
+      x = nx;
+    }
+    return r;
+  };
+  for (const CondNode& c : ct) {
+    const bool selected = c.child >= min_parent && c.child <= max_parent && is_cluster[c.child - min_parent] &&
+                          c.child != min_parent;
+    if (selected) continue;
+    const int32_t xr = ufind(c.parent), yr = ufind(c.child);
+    if (rank[xr] < rank[yr]) {
+      up[xr] = yr;
+    } else if (rank[xr] > rank[yr]) {
+      up[yr] = xr;
+    } else {
+      up[yr] = xr;
+      rank[xr] += 1;
+    }
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    const int32_t c = ufind(static_cast<int32_t>(i));
+    if (c <= min_parent) {
+      labels[i] = -1;
+    } else {
+      const float pl = lam[i], cl = lam[c];
+      labels[i] = (pl > cl && cmap[c - min_parent] >= 0) ? cmap[c - min_parent] : -1;
+    }
+  }
+  return HDB_OK;
+}

@@ -1,0 +1,739 @@
+// NN-Descent kNN-graph construction for gfx950.
+//
+// Layout in HBM (n points, K = 32):
+//   lists   n x 32 hdb_nnd_element {float d2, int label}, each list sorted by
+//           the key (d2 bits, id, old-before-new); label < 0 marks "old"
+//           (tools/nndescent_element.cuh:13-18).
+//   fwd_*   n x 16 int32 forward samples, rev_* n x 16 reverse samples,
+//           counts4 = {fwd_new_n, fwd_old_n, rev_new_cnt, rev_old_cnt} x n.
+//   bucket  n x cap u64 insert proposals (key with new flag), bucket_cnt n.
+//   worst   n f32: d2 of entry 31 of each list (insert prefilter).
+//
+// One iteration (gpuknn/nndescent.cu:2133-2166) is three launches:
+//   k_nnd_sample  PrepareGraph + PrepareReverseGraph (:44-98, :293-346)
+//   k_nnd_join    ShrinkGraph + NewNeighborsCompareKernel +
+//                 TiledNewOldNeighborsCompareKernel (:405-510, :1030-1326):
+//                 one wave64 per point; each candidate keeps its single
+//                 nearest partner (NEIGHB_CACHE_NUM = 1) and proposes it.
+//   k_nnd_update  MarkAllToOld + InsertToGlobalGraph (:1398-1403, :969-1028):
+//                 one wave per list merges the list with its proposals by a
+//                 128-key bitonic sort; no locks.
+#include <algorithm>
+
+#include "hdb_internal.hpp"
+
+namespace hdb {
+
+constexpr int kSMax = 16;  // sample list stride (SAMPLE_NUM <= 16)
+
+struct NndBufs {
+  int32_t* fwd_new;
+  int32_t* fwd_old;
+  int32_t* rev_new;
+  int32_t* rev_old;
+  int32_t* fwd_new_n;
+  int32_t* fwd_old_n;
+  int32_t* rev_new_cnt;
+  int32_t* rev_old_cnt;
+  uint64_t* bucket;
+  int32_t* bucket_cnt;
+  float* worst;
+  unsigned long long* stats;  // [0] proposals, [1] bucket drops, [2] reverse drops
+};
+
+__device__ __forceinline__ hdb_nnd_element key_to_elem(uint64_t key) {
+  hdb_nnd_element e;
+  if (key == kKeyMax) {
+    e.distance = __uint_as_float(0x7F800000u);
+    e.label = static_cast<int32_t>(kInvalidId);
+  } else {
+    const uint32_t id = key_id(key);
+    e.distance = key_d2(key);
+    e.label = key_new(key) ? static_cast<int32_t>(id) : -static_cast<int32_t>(id) - 1;
+  }
+  return e;
+}
+
+__device__ __forceinline__ float point_d2(const float* __restrict__ X, int d, uint32_t a, uint32_t b) {
+  return d2_any(X + static_cast<int64_t>(a) * d, X + static_cast<int64_t>(b) * d, d);
+}
+
+// ------------------------------------------------------------------ init --
+// InitKNNGraphIndexKernel (nndescent.cu:1673-1692) with the xorshift64* index
+// path selected at :2118, then InitKNNGraphDistanceKernel (:1694-1726) and a
+// sort of each list (SortEachBlockKernel :1728-1750).  Lane j of list p
+// draws label = xorshift64star(p*K + j) mod n and steps +1 past p.
+__global__ __launch_bounds__(256) void k_nnd_init_random(const float* __restrict__ X, int64_t n, int d,
+                                                         hdb_nnd_element* __restrict__ lists,
+                                                         float* __restrict__ worst) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t p = t >> 5;
+  const int j = static_cast<int>(t & 31);
+  uint64_t key = kKeyMax;
+  if (p < n) {
+    uint64_t lab = xorshift64star(static_cast<uint64_t>(p * kK + j)) % static_cast<uint64_t>(n);
+    if (static_cast<int64_t>(lab) == p) lab = (lab + 1) % static_cast<uint64_t>(n);
+    const float d2 = point_d2(X, d, static_cast<uint32_t>(p), static_cast<uint32_t>(lab));
+    key = make_key(d2, static_cast<uint32_t>(lab), 1u);
+  }
+  key = bitonic_group<32>(key);
+  if (p < n) {
+    lists[p * kK + j] = key_to_elem(key);
+    if (j == kK - 1) worst[p] = key_d2(key);
+  }
+}
+
+// Recompute the distances of caller-supplied lists, sort them and mark
+// every entry new (start of gpuknn::NNDescent from given lists).
+__global__ __launch_bounds__(256) void k_nnd_init_given(const float* __restrict__ X, int64_t n, int d,
+                                                        hdb_nnd_element* __restrict__ lists,
+                                                        float* __restrict__ worst) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t p = t >> 5;
+  const int j = static_cast<int>(t & 31);
+  uint64_t key = kKeyMax;
+  if (p < n) {
+    const uint32_t id = label_id(lists[p * kK + j].label);
+    if (id != kInvalidId && static_cast<int64_t>(id) < n && static_cast<int64_t>(id) != p) {
+      key = make_key(point_d2(X, d, static_cast<uint32_t>(p), id), id, 1u);
+    }
+  }
+  key = bitonic_group<32>(key);
+  // duplicates: keep the first of equal keys
+  uint32_t lo = static_cast<uint32_t>(key), hi = static_cast<uint32_t>(key >> 32);
+  uint32_t plo = __shfl_up(lo, 1), phi = __shfl_up(hi, 1);
+  const uint64_t prev = (static_cast<uint64_t>(phi) << 32) | plo;
+  const bool keep = key != kKeyMax && (j == 0 || (prev >> 1) != (key >> 1));
+  const uint64_t bal = __ballot(keep);
+  const uint32_t half = static_cast<uint32_t>(bal >> (threadIdx.x & 32));
+  const int pos = __popc(half & ((1u << j) - 1u));
+  const int total = __popc(half);
+  if (p < n) {
+    if (keep) lists[p * kK + pos] = key_to_elem(key);
+    if (j >= total) lists[p * kK + j] = key_to_elem(kKeyMax);
+    if (keep && pos == kK - 1) worst[p] = key_d2(key);
+    if (j == 0 && total < kK) worst[p] = __uint_as_float(0x7F800000u);
+  }
+}
+
+// knnmerge init: InitRandomBlockedKNNGraph (gpuknn/knnmerge.cu:80-185) after
+// MarkAllToOld of both inputs (:487-503).  List p keeps the first 16 entries
+// of its own graph (old; ids of graph 2 shifted by n1) and draws 16 new
+// cross-half ids xorshift64star(local*16 + t) mod n_other (+ n1 for lists of
+// graph 1), t = 0..15; distances are then recomputed and the list sorted
+// (InitRandomKNNGraph with start_from_random_index = false, :2015).
+__global__ __launch_bounds__(256) void k_nnd_init_merge(const float* __restrict__ X, int64_t n1, int64_t n2, int d,
+                                                        const hdb_nnd_element* __restrict__ g1,
+                                                        const hdb_nnd_element* __restrict__ g2,
+                                                        hdb_nnd_element* __restrict__ lists,
+                                                        float* __restrict__ worst) {
+  const int64_t n = n1 + n2;
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t p = t >> 5;
+  const int j = static_cast<int>(t & 31);
+  uint64_t key = kKeyMax;
+  if (p < n) {
+    const bool first = p < n1;
+    const int64_t local = first ? p : p - n1;
+    uint32_t id;
+    uint32_t is_new;
+    if (j < kK / 2) {
+      const hdb_nnd_element e = first ? g1[local * kK + j] : g2[local * kK + j];
+      id = label_id(e.label);
+      if (id != kInvalidId && static_cast<int64_t>(id) >= (first ? n1 : n2)) id = kInvalidId;
+      if (id != kInvalidId && !first) id += static_cast<uint32_t>(n1);
+      is_new = 0u;
+    } else {
+      const uint64_t r = xorshift64star(static_cast<uint64_t>(local * (kK / 2) + (j - kK / 2)));
+      id = first ? static_cast<uint32_t>(r % static_cast<uint64_t>(n2) + static_cast<uint64_t>(n1))
+                 : static_cast<uint32_t>(r % static_cast<uint64_t>(n1));
+      is_new = 1u;
+    }
+    if (id != kInvalidId) key = make_key(point_d2(X, d, static_cast<uint32_t>(p), id), id, is_new);
+  }
+  key = bitonic_group<32>(key);
+  if (p < n) {
+    lists[p * kK + j] = key_to_elem(key);
+    if (j == kK - 1) worst[p] = (key == kKeyMax) ? __uint_as_float(0x7F800000u) : key_d2(key);
+  }
+}
+
+// -------------------------------------------------------------- sampling --
+// Forward samples: the first S new and the first S old entries of the list
+// in list order (the reference keeps the first S that win an LDS atomic);
+// in merge mode old entries are kept only from the list's own half
+// (PrepareGraphForMerge :100-166).  Each forward sample q appends p to q's
+// reverse list through a counter, dropping appends beyond S
+// (PrepareReverseGraph :293-346).
+template <bool MERGE>
+__global__ __launch_bounds__(256) void k_nnd_sample(const hdb_nnd_element* __restrict__ lists, int64_t n, int S,
+                                                    int64_t split, NndBufs b) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t p = t >> 5;
+  const unsigned j = threadIdx.x & 31u;
+  bool take_new = false, take_old = false;
+  uint32_t id = kInvalidId;
+  if (p < n) {
+    const int32_t lab = lists[p * kK + j].label;
+    id = label_id(lab);
+    if (id != kInvalidId && static_cast<int64_t>(id) < n) {
+      if (lab >= 0) {
+        take_new = true;
+      } else {
+        take_old = !MERGE || ((static_cast<int64_t>(id) < split) == (p < split));
+      }
+    }
+  }
+  const unsigned sh = threadIdx.x & 32u;
+  const uint32_t mnew = static_cast<uint32_t>(__ballot(take_new) >> sh);
+  const uint32_t mold = static_cast<uint32_t>(__ballot(take_old) >> sh);
+  const uint32_t lt = (1u << j) - 1u;
+  int drops = 0;
+  if (take_new) {
+    const int r = __popc(mnew & lt);
+    if (r < S) {
+      b.fwd_new[p * kSMax + r] = static_cast<int32_t>(id);
+      const int slot = atomicAdd(&b.rev_new_cnt[id], 1);
+      if (slot < S) b.rev_new[static_cast<int64_t>(id) * kSMax + slot] = static_cast<int32_t>(p);
+      else drops++;
+    }
+  }
+  if (take_old) {
+    const int r = __popc(mold & lt);
+    if (r < S) {
+      b.fwd_old[p * kSMax + r] = static_cast<int32_t>(id);
+      const int slot = atomicAdd(&b.rev_old_cnt[id], 1);
+      if (slot < S) b.rev_old[static_cast<int64_t>(id) * kSMax + slot] = static_cast<int32_t>(p);
+      else drops++;
+    }
+  }
+  if (p < n && j == 0) {
+    b.fwd_new_n[p] = min(__popc(mnew), S);
+    b.fwd_old_n[p] = min(__popc(mold), S);
+  }
+  const uint64_t dm = __ballot(drops > 0);
+  if (__lane_id() == 0 && dm) atomicAdd(&b.stats[2], static_cast<unsigned long long>(__popcll(dm)));
+}
+
+// ------------------------------------------------------------ local join --
+// Per point p (one wave): new = sort/dedup(fwd_new + rev_new), old =
+// sort/dedup(fwd_old + rev_old), new -= old (ShrinkGraph :405-510).
+// Candidate c (lane c) computes the squared distance to each partner -- the
+// other new candidates and the old ones if c is new (NEWNEW), only the old
+// ones if c is new in merge mode, the new ones if c is old -- keeps the
+// nearest (GetMinElement/2/3 :852-961) and proposes it to c's list unless it
+// cannot beat c's current 32nd entry.
+template <int D, int WPB, bool NEWNEW>
+__global__ __launch_bounds__(WPB * 64) void k_nnd_join(const float* __restrict__ X, int64_t n, int d, int S,
+                                                       NndBufs b, int cap) {
+  constexpr bool kOwnReg = (D > 0 && D <= 32);
+  constexpr int kStride = kOwnReg ? ((D + 3) & ~3) : 0;
+  extern __shared__ float lds_dyn[];
+  __shared__ uint32_t s_new[WPB][64];
+  __shared__ uint32_t s_old[WPB][64];
+  __shared__ uint32_t s_cand[WPB][64];
+  const unsigned lane = __lane_id();
+  const unsigned w = threadIdx.x >> 6;
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * WPB + w;
+  const bool live = p < n;
+  const int dd = D > 0 ? D : d;
+  const int stride = kOwnReg ? kStride : dd + 1;
+  float* vec = lds_dyn + static_cast<int64_t>(w) * 64 * stride;
+
+  uint32_t v = 0xFFFFFFFFu;
+  if (live) {
+    const int part = lane >> 4, r = lane & 15;
+    int cnt;
+    const int32_t* src;
+    if (part == 0) { cnt = b.fwd_new_n[p]; src = b.fwd_new; }
+    else if (part == 1) { cnt = min(b.rev_new_cnt[p], S); src = b.rev_new; }
+    else if (part == 2) { cnt = b.fwd_old_n[p]; src = b.fwd_old; }
+    else { cnt = min(b.rev_old_cnt[p], S); src = b.rev_old; }
+    if (r < cnt) v = static_cast<uint32_t>(src[p * kSMax + r]);
+  }
+  v = bitonic_group_u32<32>(v);
+  const uint32_t prev = __shfl_up(v, 1);
+  const bool keep = v != 0xFFFFFFFFu && ((lane & 31u) == 0 || v != prev);
+  const uint64_t bal = __ballot(keep);
+  const uint32_t half = static_cast<uint32_t>(bal >> (lane & 32u));
+  const int pos = __popc(half & ((1u << (lane & 31u)) - 1u));
+  if (keep) {
+    if (lane < 32) s_new[w][pos] = v;
+    else s_old[w][pos] = v;
+  }
+  const int nn0 = __popc(static_cast<uint32_t>(bal));
+  const int no = __popc(static_cast<uint32_t>(bal >> 32));
+  __syncthreads();
+  // new -= old (binary search in the sorted old ids)
+  bool keep_new = false;
+  uint32_t x = 0;
+  if (lane < static_cast<unsigned>(nn0)) {
+    x = s_new[w][lane];
+    int lo = 0, hi = no;
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (s_old[w][mid] < x) lo = mid + 1; else hi = mid;
+    }
+    keep_new = !(lo < no && s_old[w][lo] == x);
+  }
+  const uint64_t bal2 = __ballot(keep_new);
+  const int nn = __popcll(bal2);
+  const int nc = nn + no;
+  if (keep_new) s_cand[w][__popcll(bal2 & lanemask_lt())] = x;
+  if (lane < static_cast<unsigned>(no)) s_cand[w][nn + lane] = s_old[w][lane];
+  __syncthreads();
+
+  const bool active = lane < static_cast<unsigned>(nc);
+  const uint32_t my_id = active ? s_cand[w][lane] : 0u;
+  float own[kOwnReg ? kStride : 1];
+  if (kOwnReg) {
+    if (active) {
+      const float* src = X + static_cast<int64_t>(my_id) * D;
+#pragma unroll
+      for (int k = 0; k < D; ++k) own[k] = src[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < D; ++k) own[k] = 0.0f;
+    }
+#pragma unroll
+    for (int k = D; k < kStride; ++k) own[k] = 0.0f;
+    if (active) {
+#pragma unroll
+      for (int k = 0; k < kStride; ++k) vec[lane * kStride + k] = own[k];
+    }
+  } else {
+    for (int c = 0; c < nc; ++c) {
+      const float* src = X + static_cast<int64_t>(s_cand[w][c]) * dd;
+      for (int k = lane; k < dd; k += 64) vec[c * stride + k] = src[k];
+    }
+  }
+  __syncthreads();
+
+  // partner range of this lane
+  int jb, je;
+  if (static_cast<int>(lane) < nn) {
+    jb = NEWNEW ? 0 : nn;
+    je = nc;
+  } else {
+    jb = 0;
+    je = nn;
+  }
+  uint64_t best = kKeyMax;
+  for (int j = 0; j < nc; ++j) {
+    const float* row = vec + j * stride;
+    float acc;
+    if (kOwnReg) {
+      acc = 0.0f;
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        const float diff = own[k] - row[k];
+        acc = __builtin_fmaf(diff, diff, acc);
+      }
+    } else {
+      acc = d2_any(vec + lane * stride, row, dd);
+    }
+    const bool ok = active && j >= jb && j < je && j != static_cast<int>(lane);
+    const uint64_t key = make_key(acc, s_cand[w][j], 1u);
+    if (ok && key < best) best = key;
+  }
+  bool propose = false;
+  if (active && best != kKeyMax) propose = key_d2(best) < b.worst[my_id];
+  bool dropped = false;
+  if (propose) {
+    const int slot = atomicAdd(&b.bucket_cnt[my_id], 1);
+    if (slot < cap) b.bucket[static_cast<int64_t>(my_id) * cap + slot] = best;
+    else dropped = true;
+  }
+  const uint64_t pm = __ballot(propose), dm = __ballot(dropped);
+  if (lane == 0) {
+    if (pm) atomicAdd(&b.stats[0], static_cast<unsigned long long>(__popcll(pm)));
+    if (dm) atomicAdd(&b.stats[1], static_cast<unsigned long long>(__popcll(dm)));
+  }
+}
+
+// ----------------------------------------------------------- list update --
+// One wave per list: the 32 list entries (all marked old) and up to 96
+// proposals (new) are sorted as 128 keys; equal (distance, id) keys collapse
+// to the first (the old entry wins, as InsertToGlobalGraph rejects an equal
+// element); the first 32 survivors form the new list.
+template <int WPB>
+__global__ __launch_bounds__(WPB * 64) void k_nnd_update(hdb_nnd_element* __restrict__ lists, int64_t n, NndBufs b,
+                                                         int cap) {
+  const unsigned lane = __lane_id();
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * WPB + (threadIdx.x >> 6);
+  if (t >= n) return;  // whole wave exits together
+  const int cnt = min(b.bucket_cnt[t], cap);
+  const uint64_t* bk = b.bucket + t * cap;
+  uint64_t r0, r1;
+  if (lane < 32) {
+    const hdb_nnd_element e = lists[t * kK + lane];
+    const uint32_t id = label_id(e.label);
+    r0 = (id == kInvalidId) ? kKeyMax : make_key(e.distance, id, 0u);
+  } else {
+    const int j = lane - 32;
+    r0 = j < cnt ? bk[j] : kKeyMax;
+  }
+  {
+    const int j = 32 + lane;
+    r1 = j < cnt ? bk[j] : kKeyMax;
+  }
+  bitonic128(r0, r1);
+  // predecessor of element i = r*64 + lane in sorted order
+  const uint64_t p0 = shfl_u64(r0, (lane + 63) & 63);
+  const uint64_t q1 = shfl_u64(r1, (lane + 63) & 63);
+  const uint64_t last0 = shfl_u64(r0, 63);
+  const uint64_t p1 = lane == 0 ? last0 : q1;
+  const bool keep0 = r0 != kKeyMax && (lane == 0 || (p0 >> 1) != (r0 >> 1));
+  const bool keep1 = r1 != kKeyMax && ((p1 >> 1) != (r1 >> 1));
+  const uint64_t b0 = __ballot(keep0), b1 = __ballot(keep1);
+  const uint64_t lt = lanemask_lt();
+  const int pos0 = __popcll(b0 & lt);
+  const int pos1 = __popcll(b0) + __popcll(b1 & lt);
+  const int total = __popcll(b0) + __popcll(b1);
+  hdb_nnd_element* out = lists + t * kK;
+  if (keep0 && pos0 < kK) out[pos0] = key_to_elem(r0);
+  if (keep1 && pos1 < kK) out[pos1] = key_to_elem(r1);
+  if (total < kK && static_cast<int>(lane) >= total && lane < static_cast<unsigned>(kK)) out[lane] = key_to_elem(kKeyMax);
+  if (keep0 && pos0 == kK - 1) b.worst[t] = key_d2(r0);
+  if (keep1 && pos1 == kK - 1) b.worst[t] = key_d2(r1);
+  if (lane == 0) {
+    if (total < kK) b.worst[t] = __uint_as_float(0x7F800000u);
+    b.bucket_cnt[t] = 0;
+    b.rev_new_cnt[t] = 0;
+    b.rev_old_cnt[t] = 0;
+  }
+}
+
+// Strip old flags so exported labels are >= 0.
+__global__ void k_nnd_finalize(hdb_nnd_element* __restrict__ lists, int64_t total) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < total) {
+    const int32_t lab = lists[i].label;
+    if (lab < 0) lists[i].label = -lab - 1;
+  }
+}
+
+// --------------------------------------------------------- unique merge --
+// UpdateKNNGraphKernel / UniqueMergeSequential (gpuknn/gen_large_knngraph.cu:
+// 136-213) with PreProcIDKernel (:215-228): running list = first 32 distinct
+// of merge(running, pair result remapped to global ids).
+__global__ __launch_bounds__(256) void k_unique_merge(hdb_nnd_element* __restrict__ running,
+                                                      const hdb_nnd_element* __restrict__ pair, int64_t n_first,
+                                                      int64_t rows, int64_t row0_pair, int64_t off_a,
+                                                      int64_t off_b) {
+  const unsigned lane = __lane_id();
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (r >= rows) return;
+  uint64_t key;
+  if (lane < 32) {
+    const hdb_nnd_element e = running[r * kK + lane];
+    const uint32_t id = label_id(e.label);
+    key = id == kInvalidId ? kKeyMax : make_key(e.distance, id, 0u);
+  } else {
+    const hdb_nnd_element e = pair[(row0_pair + r) * kK + (lane - 32)];
+    uint32_t id = label_id(e.label);
+    if (id == kInvalidId) {
+      key = kKeyMax;
+    } else {
+      const int64_t g = (static_cast<int64_t>(id) < n_first) ? off_a + id : off_b + (id - n_first);
+      key = make_key(e.distance, static_cast<uint32_t>(g), 0u);
+    }
+  }
+  key = bitonic_group<64>(key);
+  const uint64_t prev = shfl_u64(key, (lane + 63) & 63);
+  const bool keep = key != kKeyMax && (lane == 0 || (prev >> 1) != (key >> 1));
+  const uint64_t bal = __ballot(keep);
+  const int pos = __popcll(bal & lanemask_lt());
+  const int total = __popcll(bal);
+  hdb_nnd_element* out = running + r * kK;
+  if (keep && pos < kK) out[pos] = key_to_elem(key);
+  if (total < kK && static_cast<int>(lane) >= total && lane < static_cast<unsigned>(kK)) out[lane] = key_to_elem(kKeyMax);
+}
+
+__global__ void k_offset_labels(const hdb_nnd_element* __restrict__ in, hdb_nnd_element* __restrict__ out,
+                                int64_t total, int64_t off) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < total) {
+    hdb_nnd_element e = in[i];
+    const uint32_t id = label_id(e.label);
+    e.label = id == kInvalidId ? static_cast<int32_t>(kInvalidId) : static_cast<int32_t>(id + off);
+    out[i] = e;
+  }
+}
+
+// ------------------------------------------------------------ host side --
+static int get_bufs(hdb_ctx* ctx, int64_t n, int cap, NndBufs* b) {
+  HDB_CALL(scratch_t(ctx, kSlotFwdNew, n * kSMax, &b->fwd_new));
+  HDB_CALL(scratch_t(ctx, kSlotFwdOld, n * kSMax, &b->fwd_old));
+  HDB_CALL(scratch_t(ctx, kSlotRevNew, n * kSMax, &b->rev_new));
+  HDB_CALL(scratch_t(ctx, kSlotRevOld, n * kSMax, &b->rev_old));
+  int32_t* c4;
+  HDB_CALL(scratch_t(ctx, kSlotCounts4, 4 * n, &c4));
+  b->fwd_new_n = c4;
+  b->fwd_old_n = c4 + n;
+  b->rev_new_cnt = c4 + 2 * n;
+  b->rev_old_cnt = c4 + 3 * n;
+  HDB_CALL(scratch_t(ctx, kSlotBucket, n * cap, &b->bucket));
+  HDB_CALL(scratch_t(ctx, kSlotBucketCnt, n, &b->bucket_cnt));
+  HDB_CALL(scratch_t(ctx, kSlotWorst, n, &b->worst));
+  HDB_CALL(scratch_t(ctx, kSlotStats, 4, &b->stats));
+  HDB_CALL(fill_i32(ctx, c4, 4 * n, 0));
+  HDB_CALL(fill_i32(ctx, b->bucket_cnt, n, 0));
+  HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(b->stats), 4, 0));
+  return HDB_OK;
+}
+
+static int read_stats(hdb_ctx* ctx, const NndBufs& b) {
+  unsigned long long s[4];
+  HDB_HIP(hipMemcpyAsync(s, b.stats, sizeof(s), hipMemcpyDeviceToHost, ctx->stream));
+  HDB_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->stats_host[0] = static_cast<int64_t>(s[0]);
+  ctx->stats_host[1] = static_cast<int64_t>(s[1]);
+  ctx->stats_host[2] = static_cast<int64_t>(s[2]);
+  return HDB_OK;
+}
+
+template <int D, int WPB, bool NEWNEW>
+static int launch_join_t(hdb_ctx* ctx, const float* X, int64_t n, int d, int S, const NndBufs& b, int cap) {
+  constexpr bool kOwnReg = (D > 0 && D <= 32);
+  const int stride = kOwnReg ? ((D + 3) & ~3) : d + 1;
+  const size_t lds = static_cast<size_t>(WPB) * 64 * stride * sizeof(float);
+  hipLaunchKernelGGL((k_nnd_join<D, WPB, NEWNEW>), dim3(grid_for(n, WPB)), dim3(WPB * 64), lds, ctx->stream, X, n, d,
+                     S, b, cap);
+  HDB_LAUNCHED();
+  return HDB_OK;
+}
+
+template <bool NEWNEW>
+static int launch_join(hdb_ctx* ctx, const float* X, int64_t n, int d, int S, const NndBufs& b, int cap) {
+  if (d == 12) return launch_join_t<12, 4, NEWNEW>(ctx, X, n, d, S, b, cap);
+  if (d == 32) return launch_join_t<32, 4, NEWNEW>(ctx, X, n, d, S, b, cap);
+  // generic path: one candidate row of d+1 floats per lane in LDS, <= 64 KiB per block
+  const size_t per_wave = static_cast<size_t>(64) * (d + 1) * sizeof(float);
+  if (4 * per_wave <= 65536) return launch_join_t<0, 4, NEWNEW>(ctx, X, n, d, S, b, cap);
+  if (2 * per_wave <= 65536) return launch_join_t<0, 2, NEWNEW>(ctx, X, n, d, S, b, cap);
+  return launch_join_t<0, 1, NEWNEW>(ctx, X, n, d, S, b, cap);
+}
+
+static int iterate(hdb_ctx* ctx, const float* X, int64_t n, int d, int S, int cap, bool merge, int64_t split,
+                   int iterations, hdb_nnd_element* lists, const NndBufs& b) {
+  for (int it = 0; it < iterations; ++it) {
+    {
+      Stage st(ctx, merge ? "nnd_sample_merge" : "nnd_sample");
+      if (merge)
+        hipLaunchKernelGGL(k_nnd_sample<true>, dim3(grid_for(n * kK, 256)), dim3(256), 0, ctx->stream, lists, n, S,
+                           split, b);
+      else
+        hipLaunchKernelGGL(k_nnd_sample<false>, dim3(grid_for(n * kK, 256)), dim3(256), 0, ctx->stream, lists, n, S,
+                           split, b);
+      HDB_LAUNCHED();
+    }
+    {
+      Stage st(ctx, merge ? "nnd_join_merge" : "nnd_join");
+      if (merge) HDB_CALL(launch_join<false>(ctx, X, n, d, S, b, cap));
+      else HDB_CALL(launch_join<true>(ctx, X, n, d, S, b, cap));
+    }
+    {
+      Stage st(ctx, merge ? "nnd_update_merge" : "nnd_update");
+      hipLaunchKernelGGL(k_nnd_update<4>, dim3(grid_for(n, 4)), dim3(256), 0, ctx->stream, lists, n, b, cap);
+      HDB_LAUNCHED();
+    }
+  }
+  return HDB_OK;
+}
+
+static int check_params(hdb_ctx* ctx, const hdb_params& prm, int64_t n, int d) {
+  if (prm.k != kK) return ctx->fail(HDB_EUNSUPPORTED, "k must be 32 (NEIGHB_NUM_PER_LIST)");
+  if (prm.sample_num < 1 || prm.sample_num > kSMax) return ctx->fail(HDB_EINVAL, "sample_num must be in [1,16]");
+  if (prm.merge_sample_num < 1 || prm.merge_sample_num > kSMax)
+    return ctx->fail(HDB_EINVAL, "merge_sample_num must be in [1,16]");
+  if (prm.bucket_cap < 1 || prm.bucket_cap > kMaxBucketCap) return ctx->fail(HDB_EINVAL, "bucket_cap must be in [1,96]");
+  if (prm.nnd_iteration < 0 || prm.merge_iteration < 0) return ctx->fail(HDB_EINVAL, "negative iteration count");
+  if (n < 2) return ctx->fail(HDB_EINVAL, "need at least 2 points");
+  if (n >= static_cast<int64_t>(kInvalidId)) return ctx->fail(HDB_EINVAL, "n too large for int32 labels");
+  if (d < 1) return ctx->fail(HDB_EINVAL, "d must be >= 1");
+  if (d > 255) return ctx->fail(HDB_EUNSUPPORTED, "d above 255 (one LDS row per candidate)");
+  return HDB_OK;
+}
+
+int nnd_run(hdb_ctx* ctx, const float* X, int64_t n, int d, const hdb_params& prm, hdb_nnd_element* lists,
+            bool init_random, int iterations) {
+  HDB_CALL(check_params(ctx, prm, n, d));
+  if (!X || !lists) return ctx->fail(HDB_EINVAL, "null pointer");
+  NndBufs b;
+  HDB_CALL(get_bufs(ctx, n, prm.bucket_cap, &b));
+  {
+    Stage st(ctx, "nnd_init");
+    if (init_random)
+      hipLaunchKernelGGL(k_nnd_init_random, dim3(grid_for(n * kK, 256)), dim3(256), 0, ctx->stream, X, n, d, lists,
+                         b.worst);
+    else
+      hipLaunchKernelGGL(k_nnd_init_given, dim3(grid_for(n * kK, 256)), dim3(256), 0, ctx->stream, X, n, d, lists,
+                         b.worst);
+    HDB_LAUNCHED();
+  }
+  HDB_CALL(iterate(ctx, X, n, d, prm.sample_num, prm.bucket_cap, false, 0, iterations, lists, b));
+  {
+    Stage st(ctx, "nnd_finalize");
+    hipLaunchKernelGGL(k_nnd_finalize, dim3(grid_for(n * kK, 256)), dim3(256), 0, ctx->stream, lists, n * kK);
+    HDB_LAUNCHED();
+  }
+  return read_stats(ctx, b);
+}
+
+int nnd_merge_pair(hdb_ctx* ctx, const float* X, int64_t n1, int64_t n2, int d, const hdb_nnd_element* g1,
+                   const hdb_nnd_element* g2, const hdb_params& prm, hdb_nnd_element* out) {
+  const int64_t n = n1 + n2;
+  HDB_CALL(check_params(ctx, prm, n, d));
+  if (n1 < 1 || n2 < 1) return ctx->fail(HDB_EINVAL, "both graphs must be non-empty");
+  if (!X || !g1 || !g2 || !out) return ctx->fail(HDB_EINVAL, "null pointer");
+  NndBufs b;
+  HDB_CALL(get_bufs(ctx, n, prm.bucket_cap, &b));
+  {
+    Stage st(ctx, "nnd_init_merge");
+    hipLaunchKernelGGL(k_nnd_init_merge, dim3(grid_for(n * kK, 256)), dim3(256), 0, ctx->stream, X, n1, n2, d, g1, g2,
+                       out, b.worst);
+    HDB_LAUNCHED();
+  }
+  HDB_CALL(iterate(ctx, X, n, d, prm.merge_sample_num, prm.bucket_cap, true, n1, prm.merge_iteration, out, b));
+  {
+    Stage st(ctx, "nnd_finalize");
+    hipLaunchKernelGGL(k_nnd_finalize, dim3(grid_for(n * kK, 256)), dim3(256), 0, ctx->stream, out, n * kK);
+    HDB_LAUNCHED();
+  }
+  return read_stats(ctx, b);
+}
+
+int nnd_blocked(hdb_ctx* ctx, const float* X, int64_t n, int d, int blocks, const hdb_params& prm,
+                hdb_nnd_element* out) {
+  HDB_CALL(check_params(ctx, prm, n, d));
+  if (blocks < 1 || static_cast<int64_t>(blocks) * 2 > n) return ctx->fail(HDB_EINVAL, "bad block count");
+  std::vector<int64_t> off(blocks + 1);
+  for (int i = 0; i <= blocks; ++i) off[i] = n * i / blocks;  // near-equal contiguous blocks
+  for (int i = 0; i < blocks; ++i)
+    if (off[i + 1] - off[i] < 2) return ctx->fail(HDB_EINVAL, "block smaller than 2 points");
+  hdb_nnd_element* local;
+  HDB_CALL(scratch_t(ctx, kSlotBlockLocal, n * kK, &local));
+  int64_t stats[3] = {0, 0, 0};
+  // BuildEachShard (gen_large_knngraph.cu:47-92): NN-Descent per block with
+  // local ids; the running lists start as the block graphs in global ids.
+  for (int i = 0; i < blocks; ++i) {
+    const int64_t ni = off[i + 1] - off[i];
+    HDB_CALL(nnd_run(ctx, X + off[i] * d, ni, d, prm, local + off[i] * kK, true, prm.nnd_iteration));
+    for (int s = 0; s < 3; ++s) stats[s] += ctx->stats_host[s];
+    hipLaunchKernelGGL(k_offset_labels, dim3(grid_for(ni * kK, 256)), dim3(256), 0, ctx->stream, local + off[i] * kK,
+                       out + off[i] * kK, ni * kK, off[i]);
+    HDB_LAUNCHED();
+  }
+  if (blocks == 1) return HDB_OK;
+  int64_t max_pair = 0;
+  for (int i = 0; i < blocks; ++i)
+    for (int j = i + 1; j < blocks; ++j) max_pair = std::max(max_pair, off[i + 1] - off[i] + off[j + 1] - off[j]);
+  hdb_nnd_element* pair;
+  float* xp;
+  HDB_CALL(scratch_t(ctx, kSlotPairGraph, max_pair * kK, &pair));
+  HDB_CALL(scratch_t(ctx, kSlotPairX, max_pair * d, &xp));
+  // MultiMerge (gen_large_knngraph.cu:249-407): every pair i < j.
+  for (int i = 0; i < blocks; ++i) {
+    for (int j = i + 1; j < blocks; ++j) {
+      const int64_t ni = off[i + 1] - off[i], nj = off[j + 1] - off[j];
+      const float* xpair = X + off[i] * d;
+      if (j != i + 1) {
+        HDB_HIP(hipMemcpyAsync(xp, X + off[i] * d, ni * d * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+        HDB_HIP(hipMemcpyAsync(xp + ni * d, X + off[j] * d, nj * d * sizeof(float), hipMemcpyDeviceToDevice,
+                               ctx->stream));
+        xpair = xp;
+      }
+      HDB_CALL(nnd_merge_pair(ctx, xpair, ni, nj, d, local + off[i] * kK, local + off[j] * kK, prm, pair));
+      for (int s = 0; s < 3; ++s) stats[s] += ctx->stats_host[s];
+      Stage st(ctx, "nnd_unique_merge");
+      hipLaunchKernelGGL(k_unique_merge, dim3(grid_for(ni, 4)), dim3(256), 0, ctx->stream, out + off[i] * kK, pair,
+                         ni, ni, static_cast<int64_t>(0), off[i], off[j]);
+      HDB_LAUNCHED();
+      hipLaunchKernelGGL(k_unique_merge, dim3(grid_for(nj, 4)), dim3(256), 0, ctx->stream, out + off[j] * kK, pair,
+                         ni, nj, ni, off[i], off[j]);
+      HDB_LAUNCHED();
+    }
+  }
+  for (int s = 0; s < 3; ++s) ctx->stats_host[s] = stats[s];
+  return HDB_OK;
+}
+
+// ------------------------------------------------------ brute-force kNN --
+// One wave per query; each lane scans a strided share of the points keeping
+// a sorted top-kq (kq <= 32) in registers, then the wave pops the global
+// top-kq from the 64 lane lists.
+template <int D>
+__global__ __launch_bounds__(256) void k_bruteforce(const float* __restrict__ X, int64_t n, int d,
+                                                    const int32_t* __restrict__ qids, int64_t m, int kq,
+                                                    int32_t* __restrict__ out_ids, float* __restrict__ out_d2) {
+  const unsigned lane = __lane_id();
+  const int64_t qi = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (qi >= m) return;
+  const uint32_t q = static_cast<uint32_t>(qids[qi]);
+  if (static_cast<int64_t>(q) >= n) {
+    for (int r = __lane_id(); r < kq; r += 64) {
+      out_ids[qi * kq + r] = -1;
+      if (out_d2) out_d2[qi * kq + r] = __uint_as_float(0x7F800000u);
+    }
+    return;
+  }
+  uint64_t top[32];
+#pragma unroll
+  for (int i = 0; i < 32; ++i) top[i] = kKeyMax;
+  const float* xq = X + static_cast<int64_t>(q) * d;
+  for (int64_t i = lane; i < n; i += 64) {
+    if (static_cast<uint32_t>(i) == q) continue;
+    const float d2 = D > 0 ? d2_fixed<(D > 0 ? D : 1)>(xq, X + i * d) : d2_any(xq, X + i * d, d);
+    uint64_t key = (static_cast<uint64_t>(__float_as_uint(d2)) << 32) | static_cast<uint32_t>(i);
+    if (key < top[31]) {
+#pragma unroll
+      for (int s = 0; s < 32; ++s) {
+        const uint64_t a = top[s];
+        const bool sw = key < a;
+        top[s] = sw ? key : a;
+        key = sw ? a : key;
+      }
+    }
+  }
+  // pop kq minima across lanes
+  int head = 0;
+  for (int r = 0; r < kq; ++r) {
+    uint64_t h = kKeyMax;
+#pragma unroll
+    for (int s = 0; s < 32; ++s)
+      if (s == head) h = top[s];
+    uint64_t mn = h;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const uint64_t oth = shfl_xor_u64(mn, o);
+      mn = oth < mn ? oth : mn;
+    }
+    if (h == mn && mn != kKeyMax) head++;
+    if (lane == 0) {
+      out_ids[qi * kq + r] = mn == kKeyMax ? -1 : static_cast<int32_t>(static_cast<uint32_t>(mn));
+      if (out_d2) out_d2[qi * kq + r] = __uint_as_float(static_cast<uint32_t>(mn >> 32));
+    }
+  }
+}
+
+int bruteforce_knn(hdb_ctx* ctx, const float* X, int64_t n, int d, const int32_t* qids, int64_t m, int kq,
+                   int32_t* out_ids, float* out_d2) {
+  if (kq < 1 || kq > 32) return ctx->fail(HDB_EINVAL, "kq must be in [1,32]");
+  if (!X || !qids || !out_ids || n < 1 || d < 1) return ctx->fail(HDB_EINVAL, "bad brute-force arguments");
+  if (m <= 0) return HDB_OK;
+  Stage st(ctx, "bruteforce_knn");
+  if (d == 12)
+    hipLaunchKernelGGL(k_bruteforce<12>, dim3(grid_for(m, 4)), dim3(256), 0, ctx->stream, X, n, d, qids, m, kq,
+                       out_ids, out_d2);
+  else if (d == 32)
+    hipLaunchKernelGGL(k_bruteforce<32>, dim3(grid_for(m, 4)), dim3(256), 0, ctx->stream, X, n, d, qids, m, kq,
+                       out_ids, out_d2);
+  else
+    hipLaunchKernelGGL(k_bruteforce<0>, dim3(grid_for(m, 4)), dim3(256), 0, ctx->stream, X, n, d, qids, m, kq,
+                       out_ids, out_d2);
+  HDB_LAUNCHED();
+  return HDB_OK;
+}
+
+}  // namespace hdb

@@ -1,0 +1,307 @@
+// Device primitives for gfx950: fills, int32->int64 exclusive scan and a
+// stable LSD radix sort (8-bit digits, wave64 match-based ranking).
+#include "hdb_internal.hpp"
+
+namespace hdb {
+
+// ---------------------------------------------------------------- fills --
+template <typename T>
+__global__ void k_fill(T* __restrict__ p, int64_t n, T v) {
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+template <typename T>
+static int fill_t(hdb_ctx* ctx, T* p, int64_t n, T v) {
+  if (n <= 0) return HDB_OK;
+  unsigned grid = grid_for(n, 256);
+  if (grid > 65536u) grid = 65536u;
+  hipLaunchKernelGGL(k_fill<T>, dim3(grid), dim3(256), 0, ctx->stream, p, n, v);
+  HDB_LAUNCHED();
+  return HDB_OK;
+}
+int fill_i32(hdb_ctx* ctx, int32_t* p, int64_t n, int32_t v) { return fill_t(ctx, p, n, v); }
+int fill_u64(hdb_ctx* ctx, uint64_t* p, int64_t n, uint64_t v) { return fill_t(ctx, p, n, v); }
+int fill_u8(hdb_ctx* ctx, uint8_t* p, int64_t n, uint8_t v) { return fill_t(ctx, p, n, v); }
+
+int read_i64(hdb_ctx* ctx, const int64_t* p, int64_t* out) {
+  HDB_HIP(hipMemcpyAsync(out, p, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+  HDB_HIP(hipStreamSynchronize(ctx->stream));
+  return HDB_OK;
+}
+int read_i32(hdb_ctx* ctx, const int32_t* p, int32_t* out) {
+  HDB_HIP(hipMemcpyAsync(out, p, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  HDB_HIP(hipStreamSynchronize(ctx->stream));
+  return HDB_OK;
+}
+
+// ----------------------------------------------------------------- scan --
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+__device__ __forceinline__ int64_t wave_incl_scan(int64_t v) {
+  const unsigned lane = __lane_id();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int64_t u = __shfl_up(v, o);
+    if (lane >= static_cast<unsigned>(o)) v += u;
+  }
+  return v;
+}
+
+// Block-wide exclusive scan of one value per thread (256 threads); returns
+// the exclusive prefix and writes the block total to *total.
+__device__ __forceinline__ int64_t block_excl_scan(int64_t v, int64_t* total) {
+  __shared__ int64_t wsum[kScanBlock / 64];
+  const unsigned lane = __lane_id();
+  const unsigned w = threadIdx.x >> 6;
+  int64_t incl = wave_incl_scan(v);
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  int64_t off = 0, tot = 0;
+#pragma unroll
+  for (unsigned i = 0; i < kScanBlock / 64; ++i) {
+    if (i < w) off += wsum[i];
+    tot += wsum[i];
+  }
+  __syncthreads();
+  *total = tot;
+  return off + incl - v;
+}
+
+__global__ void k_scan_tiles(const int32_t* __restrict__ in, int64_t n, int64_t* __restrict__ tile_sum) {
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
+  int64_t s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    int64_t idx = base + i;
+    if (idx < n) s += in[idx];
+  }
+  int64_t total;
+  block_excl_scan(s, &total);
+  if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
+}
+
+// Single-block exclusive scan of the tile sums in place.
+__global__ void k_scan_sums(int64_t* __restrict__ sums, int64_t m) {
+  __shared__ int64_t carry_s;
+  if (threadIdx.x == 0) carry_s = 0;
+  __syncthreads();
+  for (int64_t start = 0; start < m; start += kScanBlock) {
+    int64_t idx = start + threadIdx.x;
+    int64_t v = idx < m ? sums[idx] : 0;
+    int64_t total;
+    int64_t ex = block_excl_scan(v, &total);
+    int64_t carry = carry_s;
+    if (idx < m) sums[idx] = carry + ex;
+    __syncthreads();
+    if (threadIdx.x == 0) carry_s = carry + total;
+    __syncthreads();
+  }
+}
+
+__global__ void k_scan_apply(const int32_t* __restrict__ in, int64_t n, const int64_t* __restrict__ tile_off,
+                             int64_t* __restrict__ out) {
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
+  int32_t v[kScanItems];
+  int64_t s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    int64_t idx = base + i;
+    v[i] = idx < n ? in[idx] : 0;
+    s += v[i];
+  }
+  int64_t total;
+  int64_t ex = block_excl_scan(s, &total) + tile_off[blockIdx.x];
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    int64_t idx = base + i;
+    if (idx < n) out[idx] = ex;
+    ex += v[i];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanBlock - 1) out[n] = ex;
+}
+
+int scan_i32_to_i64(hdb_ctx* ctx, const int32_t* in, int64_t n, int64_t* out) {
+  if (n <= 0) {
+    HDB_CALL(fill_t<int64_t>(ctx, out, 1, 0));
+    return HDB_OK;
+  }
+  const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+  int64_t* sums;
+  HDB_CALL(scratch_t(ctx, kSlotScanTmp, tiles, &sums));
+  hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(kScanBlock), 0, ctx->stream, in, n, sums);
+  HDB_LAUNCHED();
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, ctx->stream, sums, tiles);
+  HDB_LAUNCHED();
+  hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(kScanBlock), 0, ctx->stream, in, n, sums, out);
+  HDB_LAUNCHED();
+  return HDB_OK;
+}
+
+// ----------------------------------------------------------- radix sort --
+// Tile = 4 waves x 16 chunks x 64 lanes = 4096 keys.  Within a wave, chunk c
+// holds keys [wave_base + 64c, wave_base + 64c + 64), one per lane, so the
+// rank order (chunk, lane) is the input order and the sort is stable.
+constexpr int kRsBlock = 256;
+constexpr int kRsChunks = 16;
+constexpr int kRsWaveKeys = 64 * kRsChunks;
+constexpr int kRsTile = kRsBlock * kRsChunks;
+constexpr int kRsBins = 256;
+
+template <typename K>
+__global__ void k_rs_hist(const K* __restrict__ keys, int64_t n, int shift, int64_t ntiles,
+                          int32_t* __restrict__ hist) {
+  __shared__ int32_t h[kRsBins];
+  for (int i = threadIdx.x; i < kRsBins; i += kRsBlock) h[i] = 0;
+  __syncthreads();
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kRsTile;
+  for (int i = threadIdx.x; i < kRsTile; i += kRsBlock) {
+    int64_t idx = base + i;
+    if (idx < n) {
+      unsigned dgt = static_cast<unsigned>(keys[idx] >> shift) & 0xFFu;
+      atomicAdd(&h[dgt], 1);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kRsBins; i += kRsBlock) hist[static_cast<int64_t>(i) * ntiles + blockIdx.x] = h[i];
+}
+
+// Exclusive scan of the digit-major histogram (kRsBins * ntiles int32) in
+// place, by one block.
+__global__ void k_rs_scan(int32_t* __restrict__ hist, int64_t total) {
+  __shared__ int64_t carry_s;
+  if (threadIdx.x == 0) carry_s = 0;
+  __syncthreads();
+  constexpr int kItems = 16;
+  for (int64_t start = 0; start < total; start += static_cast<int64_t>(kRsBlock) * kItems) {
+    int64_t base = start + static_cast<int64_t>(threadIdx.x) * kItems;
+    int32_t v[kItems];
+    int64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      v[i] = (base + i < total) ? hist[base + i] : 0;
+      s += v[i];
+    }
+    int64_t tot;
+    int64_t ex = block_excl_scan(s, &tot) + carry_s;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      if (base + i < total) hist[base + i] = static_cast<int32_t>(ex);
+      ex += v[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) carry_s += tot;
+    __syncthreads();
+  }
+}
+
+template <typename K, typename V, bool HAS_V>
+__global__ __launch_bounds__(kRsBlock) void k_rs_scatter(const K* __restrict__ kin, const V* __restrict__ vin,
+                                                         K* __restrict__ kout, V* __restrict__ vout, int64_t n,
+                                                         int shift, int64_t ntiles,
+                                                         const int32_t* __restrict__ hist) {
+  __shared__ int32_t wcnt[kRsBlock / 64][kRsBins];
+  __shared__ int32_t goff[kRsBins];
+  const unsigned lane = __lane_id();
+  const unsigned w = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < (kRsBlock / 64) * kRsBins; i += kRsBlock) (&wcnt[0][0])[i] = 0;
+  for (int i = threadIdx.x; i < kRsBins; i += kRsBlock) goff[i] = hist[static_cast<int64_t>(i) * ntiles + blockIdx.x];
+  __syncthreads();
+  const int64_t wbase = static_cast<int64_t>(blockIdx.x) * kRsTile + static_cast<int64_t>(w) * kRsWaveKeys;
+  K key[kRsChunks];
+  V val[kRsChunks];
+  int32_t rank[kRsChunks];
+  uint8_t dig[kRsChunks];
+  const uint64_t lt = lanemask_lt();
+#pragma unroll
+  for (int c = 0; c < kRsChunks; ++c) {
+    const int64_t idx = wbase + c * 64 + lane;
+    const bool act = idx < n;
+    key[c] = act ? kin[idx] : K(0);
+    if (HAS_V) val[c] = act ? vin[idx] : V(0);
+    const unsigned dg = static_cast<unsigned>(key[c] >> shift) & 0xFFu;
+    dig[c] = static_cast<uint8_t>(dg);
+    uint64_t m = __ballot(act);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (dg >> b) & 1u;
+      const uint64_t bal = __ballot(bit);
+      m &= bit ? bal : ~bal;
+    }
+    // Lanes of one digit: the lowest one advances the wave's digit counter.
+    const int leader = act ? __builtin_ctzll(m) : static_cast<int>(lane);
+    int32_t base = 0;
+    if (act && static_cast<int>(lane) == leader) {
+      base = wcnt[w][dg];
+      wcnt[w][dg] = base + __popcll(m);
+    }
+    base = __shfl(base, leader);
+    rank[c] = act ? base + __popcll(m & lt) : 0;
+  }
+  __syncthreads();
+  // Per digit: exclusive prefix over waves, in wave order.
+  for (int dg = threadIdx.x; dg < kRsBins; dg += kRsBlock) {
+    int32_t acc = 0;
+#pragma unroll
+    for (int ww = 0; ww < kRsBlock / 64; ++ww) {
+      int32_t c = wcnt[ww][dg];
+      wcnt[ww][dg] = acc;
+      acc += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < kRsChunks; ++c) {
+    const int64_t idx = wbase + c * 64 + lane;
+    if (idx < n) {
+      const int64_t pos = static_cast<int64_t>(goff[dig[c]]) + wcnt[w][dig[c]] + rank[c];
+      kout[pos] = key[c];
+      if (HAS_V) vout[pos] = val[c];
+    }
+  }
+}
+
+template <typename K, typename V, bool HAS_V>
+static int radix_sort_t(hdb_ctx* ctx, K* keys, V* vals, int64_t n, int bit0, int bit1) {
+  if (n <= 1) return HDB_OK;
+  const int64_t ntiles = (n + kRsTile - 1) / kRsTile;
+  K* k2;
+  V* v2 = nullptr;
+  int32_t* hist;
+  HDB_CALL(scratch_t(ctx, kSlotSortKeys, static_cast<size_t>(n), &k2));
+  if (HAS_V) HDB_CALL(scratch_t(ctx, kSlotSortVals, static_cast<size_t>(n), &v2));
+  HDB_CALL(scratch_t(ctx, kSlotSortHist, static_cast<size_t>(kRsBins * ntiles), &hist));
+  K* ka = keys;
+  K* kb = k2;
+  V* va = vals;
+  V* vb = v2;
+  int passes = 0;
+  for (int shift = bit0; shift < bit1; shift += 8, ++passes) {
+    hipLaunchKernelGGL(k_rs_hist<K>, dim3(ntiles), dim3(kRsBlock), 0, ctx->stream, ka, n, shift, ntiles, hist);
+    HDB_LAUNCHED();
+    hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(kRsBlock), 0, ctx->stream, hist, kRsBins * ntiles);
+    HDB_LAUNCHED();
+    hipLaunchKernelGGL((k_rs_scatter<K, V, HAS_V>), dim3(ntiles), dim3(kRsBlock), 0, ctx->stream, ka, va, kb, vb, n,
+                       shift, ntiles, hist);
+    HDB_LAUNCHED();
+    K* tk = ka; ka = kb; kb = tk;
+    V* tv = va; va = vb; vb = tv;
+  }
+  if (passes & 1) {
+    HDB_HIP(hipMemcpyAsync(keys, ka, static_cast<size_t>(n) * sizeof(K), hipMemcpyDeviceToDevice, ctx->stream));
+    if (HAS_V) HDB_HIP(hipMemcpyAsync(vals, va, static_cast<size_t>(n) * sizeof(V), hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  return HDB_OK;
+}
+
+int radix_sort_u32_u64(hdb_ctx* ctx, uint32_t* keys, uint64_t* vals, int64_t n, int bit0, int bit1) {
+  return radix_sort_t<uint32_t, uint64_t, true>(ctx, keys, vals, n, bit0, bit1);
+}
+int radix_sort_u64(hdb_ctx* ctx, uint64_t* keys, int64_t n, int bit0, int bit1) {
+  return radix_sort_t<uint64_t, uint64_t, false>(ctx, keys, static_cast<uint64_t*>(nullptr), n, bit0, bit1);
+}
+
+}  // namespace hdb

@@ -1,0 +1,211 @@
+/*
+ * hdbscan_mi355x.h -- C ABI of the MI355X (gfx950) HDBSCAN hot path.
+ *
+ * Path: NN-Descent kNN graph (random init, sampling, local join, list update,
+ * knnmerge and the blocked all-pairs schedule) -> hub counts -> anti-hubs ->
+ * core distances -> mutual-reachability graph -> Boruvka MST -> sorted MST
+ * edges (-> host-side labels for parity).
+ *
+ * Conventions
+ *   - Every array argument named *_dev is a device pointer (hipMalloc'd or
+ *     obtained from hdb_dev_alloc); *_host arguments are host pointers.
+ *   - The caller allocates every output; no entry point frees an input.
+ *   - Every entry point returns an hdb_status; it never exits the process.
+ *     hdb_last_error(ctx) returns a message for the last failure on ctx.
+ *   - Work is issued on the context's stream; entry points that return
+ *     sizes to the host synchronise that stream.
+ *   - One context is not thread-safe; use one context per host thread.
+ *
+ * Reference layouts kept bit-identical (tools/nndescent_element.cuh:9-57,
+ * HDBSCAN/structs/hdbscan_elements.cuh:40-45, HDBSCAN/structs/ECLgraph.h:41-47).
+ */
+#ifndef HDBSCAN_MI355X_H
+#define HDBSCAN_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  HDB_OK = 0,
+  HDB_EINVAL = 1,        /* bad argument (sizes, parameters, null pointer) */
+  HDB_ENOMEM = 2,        /* device allocation failed */
+  HDB_EDEVICE = 3,       /* HIP runtime error (message in hdb_last_error) */
+  HDB_EDISCONNECTED = 4, /* MST is a spanning forest with >1 component */
+  HDB_EUNSUPPORTED = 5   /* parameter combination not built */
+} hdb_status;
+
+/* NNDElement (tools/nndescent_element.cuh:9-57): distance_ is the SQUARED
+ * L2 distance; label_ >= 0 is a "new" entry, -label-1 an "old" one.  Lists
+ * returned by this library always carry label >= 0. */
+typedef struct {
+  float distance;
+  int32_t label;
+} hdb_nnd_element;
+
+/* MSTedge (HDBSCAN/structs/hdbscan_elements.cuh:40-45). from_node < to_node. */
+typedef struct {
+  int32_t from_node;
+  int32_t to_node;
+  float weight;
+} hdb_mst_edge;
+
+/* Runtime form of the reference's compile-time constants
+ * (gpuknn/nndescent.cuh:9-30, HDBSCAN/structs/hdbscan_elements.cuh:8-12).
+ * hdb_params_default() fills the reference values. */
+typedef struct {
+  int32_t k;                /* NEIGHB_NUM_PER_LIST, must be 32 */
+  int32_t sample_num;       /* SAMPLE_NUM (16), 1..16 */
+  int32_t nnd_iteration;    /* NND_ITERATION (6) */
+  int32_t merge_sample_num; /* MERGE_SAMPLE_NUM (12), 1..16 */
+  int32_t merge_iteration;  /* MERGE_ITERATION (11) */
+  int32_t mpts;             /* core distance = entry mpts-1 of each list (32) */
+  int32_t distance_mode;    /* 0: FP32 join; 1: BF16 MFMA join + FP32 re-rank */
+  int32_t rerank_keep;      /* BF16 mode: survivors per candidate list (48) */
+  int32_t bucket_cap;       /* list-update proposals kept per list per pass (96) */
+  int32_t reserved[7];
+} hdb_params;
+
+typedef struct hdb_ctx hdb_ctx;
+
+/* ---- context, memory and timing plumbing --------------------------------- */
+void hdb_params_default(hdb_params* p);
+int hdb_ctx_create(hdb_ctx** ctx, int device);
+int hdb_ctx_destroy(hdb_ctx* ctx);
+const char* hdb_last_error(const hdb_ctx* ctx);
+int hdb_synchronize(hdb_ctx* ctx);
+int hdb_dev_alloc(hdb_ctx* ctx, void** ptr_dev, size_t bytes);
+int hdb_dev_free(hdb_ctx* ctx, void* ptr_dev);
+int hdb_memcpy_h2d(hdb_ctx* ctx, void* dst_dev, const void* src_host, size_t bytes);
+int hdb_memcpy_d2h(hdb_ctx* ctx, void* dst_host, const void* src_dev, size_t bytes);
+int hdb_memcpy_d2d(hdb_ctx* ctx, void* dst_dev, const void* src_dev, size_t bytes);
+/* Stage timer: when enabled, every entry point records HIP events around its
+ * kernels on the context stream; hdb_timer_read returns per-stage totals (ms)
+ * and per-kernel launch counts since the last reset. */
+int hdb_timer_enable(hdb_ctx* ctx, int enable);
+int hdb_timer_reset(hdb_ctx* ctx);
+int hdb_timer_read(hdb_ctx* ctx, const char** names, double* ms, int64_t* launches,
+                   int max_entries, int* n_entries);
+/* Counters of the last NN-Descent run: proposals made, proposals dropped at
+ * full list buckets, reverse samples dropped at full reverse lists. */
+int hdb_nnd_stats(hdb_ctx* ctx, int64_t* proposals, int64_t* bucket_drops,
+                  int64_t* reverse_drops);
+
+/* ---- kNN graph (gpuknn/) ---------------------------------------------------- */
+/* Replaces gpuknn::NNDescent (gpuknn/nndescent.cuh:40-42, nndescent.cu:2088-2199):
+ * random xorshift64* init + params->nnd_iteration NN-Descent iterations over
+ * X_dev (n x d row-major fp32).  out_dev: n x k lists sorted by (distance, label). */
+int hdb_nndescent(hdb_ctx* ctx, const float* X_dev, int64_t n, int32_t d,
+                  const hdb_params* params, hdb_nnd_element* out_dev);
+
+/* Same iterations starting from caller-supplied lists (the init step of
+ * gpuknn::NNDescent replaced; used by the learned initialiser).  Labels in
+ * lists_dev must be in [0, n) and != own index; distances are recomputed. */
+int hdb_nndescent_from(hdb_ctx* ctx, const float* X_dev, int64_t n, int32_t d,
+                       const hdb_params* params, int32_t iterations,
+                       hdb_nnd_element* lists_dev);
+
+/* Replaces gpuknn::KNNMerge / KNNMergeFromHost (gpuknn/knnmerge.cu:401-437,
+ * 478-530): X_dev holds the n1 points of graph 1 followed by the n2 points of
+ * graph 2; g1_dev (n1 x k, ids in [0,n1)) and g2_dev (n2 x k, ids in [0,n2)).
+ * out_dev (n1+n2) x k receives the merged graph with ids in [0, n1+n2). */
+int hdb_knn_merge(hdb_ctx* ctx, const float* X_dev, int64_t n1, int64_t n2, int32_t d,
+                  const hdb_nnd_element* g1_dev, const hdb_nnd_element* g2_dev,
+                  const hdb_params* params, hdb_nnd_element* out_dev);
+
+/* Replaces GenLargeKNNGraph (gpuknn/gen_large_knngraph.cu:409-484) for one
+ * device: `blocks` contiguous blocks, NN-Descent per block, knnmerge of every
+ * pair i<j and unique-merge of each pair result into the running lists
+ * (MultiMerge :249-407).  out_dev: n x k with global ids. */
+int hdb_build_knng_blocked(hdb_ctx* ctx, const float* X_dev, int64_t n, int32_t d,
+                           int32_t blocks, const hdb_params* params,
+                           hdb_nnd_element* out_dev);
+
+/* Exact brute-force kNN (FP32, FP64-free) for m query ids against all n
+ * points; replaces knn_cuda_global (gpuknn/knncuda.cu:274) in its role of
+ * sampled recall check.  out_ids_dev: m x kq (ascending distance, self
+ * excluded), out_d2_dev may be NULL. */
+int hdb_bruteforce_knn(hdb_ctx* ctx, const float* X_dev, int64_t n, int32_t d,
+                       const int32_t* query_ids_dev, int64_t m, int32_t kq,
+                       int32_t* out_ids_dev, float* out_d2_dev);
+
+/* ---- HDBSCAN graph stage (HDBSCAN/counts, calculates, graphs) -------------- */
+/* Replaces calculate_degrees/countDegrees (calculates.cu:448-473,
+ * count.cu:36-108): counts_dev[v] = number of list entries equal to v. */
+int hdb_hub_counts(hdb_ctx* ctx, const hdb_nnd_element* lists_dev, int64_t n, int32_t k,
+                   int32_t* counts_dev);
+
+/* Replaces the anti-hub selection of buildEnhancedKNNG (graph.cu:108-151,
+ * calculates.cu:491-523): the floor(sqrt(n)) vertices of lowest in-degree,
+ * ties at the threshold degree by lowest score (sum of the in-degrees of the
+ * vertex's list), then lowest id; returned sorted by id in ids_dev (capacity
+ * floor(sqrt(n))).  *h_out receives the count. */
+int hdb_antihubs(hdb_ctx* ctx, const hdb_nnd_element* lists_dev, const int32_t* counts_dev,
+                 int64_t n, int32_t k, int32_t* ids_dev, int64_t* h_out);
+
+/* Replaces calculateCoreDistance (calculates.cu:185-197, index mpts-1 as at
+ * graph.cu:77): core_dev[i] = Euclidean distance from i to entry mpts-1 of
+ * its list, recomputed in FP64 and rounded once to FP32. */
+int hdb_core_distances(hdb_ctx* ctx, const float* X_dev, const hdb_nnd_element* lists_dev,
+                       int64_t n, int32_t d, int32_t k, int32_t mpts, float* core_dev);
+
+/* Replaces buildECLgraph (graph.cu:16-103) with calculateMRD
+ * (calculates.cu:61-82) and the anti-hub clique weights (calculates.cu:412-446).
+ * Symmetric CSR of the kNN graph plus the anti-hub clique, rows sorted by
+ * neighbour id (a kNN entry before a clique entry of the same neighbour).
+ * Weights: max(core_a, core_b) for kNN edges, max(core_a, core_b, d_ab) for
+ * clique edges.  Two-phase: call with nlist_dev == NULL to get *edges_out
+ * (nindex_dev is filled in both phases), then with nlist_dev/w_dev of that size. */
+int hdb_mrd_graph(hdb_ctx* ctx, const float* X_dev, const hdb_nnd_element* lists_dev,
+                  const float* core_dev, const int32_t* antihubs_dev, int64_t h,
+                  int64_t n, int32_t d, int32_t k, int64_t* nindex_dev,
+                  int32_t* nlist_dev, float* w_dev, int64_t* edges_out);
+
+/* ---- MST and edge sort (HDBSCAN/mst, HDBSCAN/merge_sort) ------------------- */
+/* Replaces gpuMST + buildMST (mst.cu:210-334): minimum spanning forest of the
+ * CSR graph under the strict order (weight, CSR position of the from<to
+ * entry), i.e. (weight, min id, max id).  out_dev (capacity n-1) receives the
+ * edges in CSR order; *m_out the edge count; *components_out the number of
+ * trees.  Returns HDB_EDISCONNECTED (with valid outputs) when components > 1. */
+int hdb_mst(hdb_ctx* ctx, const int64_t* nindex_dev, const int32_t* nlist_dev,
+            const float* w_dev, int64_t n, hdb_mst_edge* out_dev, int64_t* m_out,
+            int64_t* components_out);
+
+/* Replaces sort_edges (merge_sort.cu:226-279): stable ascending sort of m
+ * edges by weight, in place. */
+int hdb_sort_edges(hdb_ctx* ctx, hdb_mst_edge* edges_dev, int64_t m);
+
+/* ---- whole path ------------------------------------------------------------- */
+typedef struct {
+  int32_t knng_mode;      /* 0: one NN-Descent over all points; 1: blocked all-pairs */
+  int32_t blocks;         /* knng_mode 1: number of blocks */
+  int32_t keep_csr;       /* nonzero: leave the MRD CSR in the context (testing) */
+  int32_t reserved[5];
+} hdb_pipeline_opts;
+
+/* kNNG -> hub counts -> anti-hubs -> core distances -> MRD graph -> MST ->
+ * sorted MST edges, all on device.  Outputs (device, caller-allocated):
+ * lists_dev n x k, counts_dev n, core_dev n, edges_dev n-1 (sorted by
+ * (weight, from, to)).  *m_out: MST edge count; *components_out: tree count.
+ * Mirrors the call order of HDBSCAN/_new.cu:149-314. */
+int hdb_pipeline(hdb_ctx* ctx, const float* X_dev, int64_t n, int32_t d,
+                 const hdb_params* params, const hdb_pipeline_opts* opts,
+                 hdb_nnd_element* lists_dev, int32_t* counts_dev, float* core_dev,
+                 hdb_mst_edge* edges_dev, int64_t* m_out, int64_t* components_out,
+                 int64_t* csr_edges_out);
+
+/* ---- host labels (HDBSCAN/trees/tree.cu) ----------------------------------- */
+/* Single linkage -> condensed tree (min cluster size mpts) -> stability ->
+ * EOM selection -> labels, following tree.cu:171-746 on host memory.
+ * edges_host: n-1 MST edges sorted by weight.  labels_host: n (noise = -1).
+ * *clusters_out: number of selected clusters. */
+int hdb_labels(const hdb_mst_edge* edges_host, int64_t n, int32_t mpts,
+               int32_t* labels_host, int32_t* clusters_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HDBSCAN_MI355X_H */

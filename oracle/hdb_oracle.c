@@ -1,0 +1,460 @@
+/*
+ * hdb_oracle.c -- CPU oracle for the HDBSCAN hot path (TEST INFRASTRUCTURE).
+ *
+ * This file is the checker, never the product: only tests/, the smoke check
+ * and the CPU-baseline leg of bench.py load it.  It restates, in plain C with
+ * OpenMP, the reference algorithms of gorlando04/GPU_HDBSCAN for the path
+ * (citations are file:line in the reference tree):
+ *
+ *   or_xorshift64star    gpuknn/knncuda_tools.cu:126-132
+ *   or_init_random       gpuknn/nndescent.cu:1673-1750 (xorshift index path, :2118)
+ *   or_d2                squared L2, FP32, dimension order, fused multiply-add
+ *   or_bruteforce_fp64   exact kNN in FP64 (recall ground truth; role of
+ *                        gpuknn/knncuda.cu:274 knn_cuda_global)
+ *   or_recall            tools/evaluate.hpp:13-52 (set intersection of top-r)
+ *   or_hub_counts        HDBSCAN/counts/count.cu:10-19, 36-108
+ *   or_antihubs          HDBSCAN/graphs/graph.cu:108-151, getters.cu:6-39,
+ *                        calculates.cu:21-36, 491-523 (ties: score, then id)
+ *   or_core              calculates.cu:185-197 at index mpts-1 (graph.cu:77),
+ *                        Euclidean, FP64 accumulation rounded once
+ *   or_mrd_csr           calculates.cu:286-409 (nindex with findKNNlist flags,
+ *                        nlist own/reverse/clique), calculateMRD :61-82,
+ *                        calculate_coreDistance_antihubs :412-446; rows then
+ *                        sorted by neighbour id, kNN entry before clique entry
+ *   or_kruskal           HDBSCAN/mst/mst.cu:7-78 (cpuMST) + buildMST :291-334
+ *   or_sort_edges        HDBSCAN/merge_sort/merge_sort.cu:27-116 (stable by weight)
+ *
+ * Parity pins for each function are in tests/ (golden vectors, closed forms,
+ * brute force on tiny inputs); see the per-function notes there.
+ *
+ * Compiled with -ffp-contract=off so FP64 sums and FP32 products round like
+ * the written expression; or_d2 uses fmaf explicitly.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct {
+  float distance;
+  int32_t label;
+} or_elem;
+
+typedef struct {
+  int32_t from_node;
+  int32_t to_node;
+  float weight;
+} or_edge;
+
+#define OR_INVALID 0x7FFFFFFF
+
+uint64_t or_xorshift64star(uint64_t x) {
+  x += 42ull;
+  x ^= x >> 12;
+  x ^= x << 25;
+  x ^= x >> 27;
+  return x * 0x2545F4914F6CDD1Dull;
+}
+
+float or_d2(const float* a, const float* b, int d) {
+  float acc = 0.0f;
+  for (int k = 0; k < d; ++k) {
+    float diff = a[k] - b[k];
+    acc = fmaf(diff, diff, acc);
+  }
+  return acc;
+}
+
+static double d2_fp64(const float* a, const float* b, int d) {
+  double acc = 0.0;
+  for (int k = 0; k < d; ++k) {
+    double diff = (double)a[k] - (double)b[k];
+    acc = acc + diff * diff;
+  }
+  return acc;
+}
+
+float or_dist_fp64(const float* a, const float* b, int d) { return (float)sqrt(d2_fp64(a, b, d)); }
+
+static uint32_t f2u(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  return u;
+}
+
+/* list key: (d2 bits, id, new flag) as in the product */
+static uint64_t list_key(float d2, uint32_t id, uint32_t is_new) {
+  return ((uint64_t)f2u(d2) << 32) | ((uint64_t)id << 1) | is_new;
+}
+
+static int cmp_u64(const void* a, const void* b) {
+  uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+  return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+/* InitKNNGraphIndexKernel: label = xorshift64star(i*32 + j) % n, +1 past
+ * self (NEIGHB_BLOCKS_NUM = 1), distances, then each list sorted. */
+void or_init_random(const float* X, int64_t n, int d, or_elem* out) {
+  const int K = 32;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    uint64_t keys[32];
+    for (int j = 0; j < K; ++j) {
+      uint64_t lab = or_xorshift64star((uint64_t)(i * K + j)) % (uint64_t)n;
+      if ((int64_t)lab == i) lab = (lab + 1) % (uint64_t)n;
+      float d2 = or_d2(X + i * d, X + (int64_t)lab * d, d);
+      keys[j] = list_key(d2, (uint32_t)lab, 1u);
+    }
+    qsort(keys, K, sizeof(uint64_t), cmp_u64);
+    for (int j = 0; j < K; ++j) {
+      uint32_t db = (uint32_t)(keys[j] >> 32);
+      memcpy(&out[i * K + j].distance, &db, 4);
+      out[i * K + j].label = (int32_t)((uint32_t)keys[j] >> 1);
+    }
+  }
+}
+
+/* Exact kNN in FP64 for queries qids against all n points, self excluded,
+ * ascending (distance, id). */
+void or_bruteforce_fp64(const float* X, int64_t n, int d, const int32_t* qids, int64_t m, int k, int32_t* out_ids,
+                        double* out_d2) {
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int64_t qi = 0; qi < m; ++qi) {
+    const int64_t q = qids[qi];
+    double* bd = (double*)malloc(sizeof(double) * (size_t)k);
+    int32_t* bi = (int32_t*)malloc(sizeof(int32_t) * (size_t)k);
+    int cnt = 0;
+    for (int64_t i = 0; i < n; ++i) {
+      if (i == q) continue;
+      double dd = d2_fp64(X + q * d, X + i * d, d);
+      if (cnt == k && (dd > bd[k - 1] || (dd == bd[k - 1] && i > bi[k - 1]))) continue;
+      int pos = cnt < k ? cnt++ : k - 1;
+      while (pos > 0 && (bd[pos - 1] > dd || (bd[pos - 1] == dd && bi[pos - 1] > i))) {
+        bd[pos] = bd[pos - 1];
+        bi[pos] = bi[pos - 1];
+        --pos;
+      }
+      bd[pos] = dd;
+      bi[pos] = (int32_t)i;
+    }
+    for (int j = 0; j < k; ++j) {
+      out_ids[qi * k + j] = j < cnt ? bi[j] : -1;
+      if (out_d2) out_d2[qi * k + j] = j < cnt ? bd[j] : INFINITY;
+    }
+    free(bd);
+    free(bi);
+  }
+}
+
+static int cmp_i32(const void* a, const void* b) {
+  int32_t x = *(const int32_t*)a, y = *(const int32_t*)b;
+  return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+/* Evaluate (tools/evaluate.hpp:13-52): recall@r = sum |top-r(res) n top-r(gt)| / (m r). */
+double or_recall(const int32_t* res, int res_stride, const int32_t* gt, int gt_stride, int64_t m, int r) {
+  int64_t tp = 0;
+#pragma omp parallel for reduction(+ : tp)
+  for (int64_t i = 0; i < m; ++i) {
+    int32_t a[64], b[64];
+    for (int j = 0; j < r; ++j) {
+      a[j] = res[i * res_stride + j];
+      b[j] = gt[i * gt_stride + j];
+    }
+    qsort(a, r, 4, cmp_i32);
+    qsort(b, r, 4, cmp_i32);
+    int x = 0, y = 0;
+    while (x < r && y < r) {
+      if (a[x] == b[y]) {
+        tp++;
+        x++;
+        y++;
+      } else if (a[x] < b[y]) {
+        x++;
+      } else {
+        y++;
+      }
+    }
+  }
+  return (double)tp / ((double)m * (double)r);
+}
+
+static int32_t elem_id(int32_t label) { return label >= 0 ? label : -label - 1; }
+
+void or_hub_counts(const or_elem* lists, int64_t n, int k, int32_t* counts) {
+  memset(counts, 0, sizeof(int32_t) * (size_t)n);
+  for (int64_t t = 0; t < n * k; ++t) {
+    int32_t v = elem_id(lists[t].label);
+    if (v != OR_INVALID && v < n) counts[v]++;
+  }
+}
+
+typedef struct {
+  int32_t deg;
+  int64_t score;
+  int32_t id;
+} ah_vertex;
+
+static int cmp_deg_id(const void* a, const void* b) {
+  const ah_vertex* x = (const ah_vertex*)a;
+  const ah_vertex* y = (const ah_vertex*)b;
+  if (x->deg != y->deg) return x->deg < y->deg ? -1 : 1;
+  return x->id < y->id ? -1 : (x->id > y->id ? 1 : 0);
+}
+
+static int cmp_score_id(const void* a, const void* b) {
+  const ah_vertex* x = (const ah_vertex*)a;
+  const ah_vertex* y = (const ah_vertex*)b;
+  if (x->score != y->score) return x->score < y->score ? -1 : 1;
+  return x->id < y->id ? -1 : (x->id > y->id ? 1 : 0);
+}
+
+/* buildEnhancedKNNG anti-hub part: pos_threshold = (int)sqrt(n); the
+ * threshold degree is the degree of the pos_threshold-th smallest; vertices
+ * below it are taken; among those equal to it the lowest scores (sum of the
+ * in-degrees of the vertex's list, calculateScore) fill the remaining slots,
+ * ties by id; the result sorted by id. */
+void or_antihubs(const or_elem* lists, const int32_t* counts, int64_t n, int k, int32_t* ids, int64_t* h_out) {
+  int64_t h = (int64_t)sqrt((double)n);
+  while (h * h > n) --h;
+  while ((h + 1) * (h + 1) <= n) ++h;
+  *h_out = h;
+  if (h == 0) return;
+  ah_vertex* v = (ah_vertex*)malloc(sizeof(ah_vertex) * (size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    v[i].deg = counts[i];
+    v[i].score = 0;
+    v[i].id = (int32_t)i;
+  }
+  qsort(v, (size_t)n, sizeof(ah_vertex), cmp_deg_id);
+  const int32_t thr = v[h - 1].deg;
+  int64_t below = 0;
+  while (below < n && v[below].deg < thr) ++below;
+  int64_t ntied = 0;
+  ah_vertex* tied = (ah_vertex*)malloc(sizeof(ah_vertex) * (size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    if (counts[i] != thr) continue;
+    int64_t s = 0;
+    for (int j = 0; j < k; ++j) {
+      int32_t nb = elem_id(lists[i * k + j].label);
+      if (nb != OR_INVALID && nb < n) s += counts[nb];
+    }
+    if (s > 0xFFFFFFFFll) s = 0xFFFFFFFFll;
+    tied[ntied].deg = thr;
+    tied[ntied].score = s;
+    tied[ntied].id = (int32_t)i;
+    ntied++;
+  }
+  qsort(tied, (size_t)ntied, sizeof(ah_vertex), cmp_score_id);
+  int64_t c = 0;
+  for (int64_t i = 0; i < below; ++i) ids[c++] = v[i].id;
+  for (int64_t i = 0; c < h; ++i) ids[c++] = tied[i].id;
+  qsort(ids, (size_t)h, sizeof(int32_t), cmp_i32);
+  free(v);
+  free(tied);
+}
+
+void or_core(const float* X, const or_elem* lists, int64_t n, int d, int k, int mpts, float* core) {
+#pragma omp parallel for
+  for (int64_t i = 0; i < n; ++i) {
+    int32_t j = elem_id(lists[i * k + mpts - 1].label);
+    core[i] = (j == OR_INVALID || j >= n) ? INFINITY : or_dist_fp64(X + i * d, X + (int64_t)j * d, d);
+  }
+}
+
+/* findKNNlist (getters.cu:41-49): occurrences of i in the list of neig. */
+static int find_in_list(const int32_t* ids, int64_t neig, int64_t i, int k) {
+  int s = 0;
+  for (int j = 0; j < k; ++j)
+    if (ids[neig * k + j] == i) s++;
+  return s;
+}
+
+typedef struct {
+  int32_t nb;
+  int32_t is_clique;
+  float w;
+} row_entry;
+
+static int cmp_row_entry(const void* a, const void* b) {
+  const row_entry* x = (const row_entry*)a;
+  const row_entry* y = (const row_entry*)b;
+  if (x->nb != y->nb) return x->nb < y->nb ? -1 : 1;
+  return x->is_clique - y->is_clique;
+}
+
+/* Enhanced kNNG CSR (calculate_nindex / calculate_nlist, calculates.cu:286-409)
+ * followed by calculateMRD (:61-82) and the anti-hub clique distances
+ * (:412-446).  With upper != 0 only entries with nb > row are kept.  Returns
+ * the entry count; nlist/w may be NULL for a size query.  Requires lists
+ * without duplicate ids (then FLAG <= 1 everywhere). */
+int64_t or_mrd_csr(const float* X, const or_elem* lists, const float* core, const int32_t* ah, int64_t h, int64_t n,
+                   int d, int k, int upper, int64_t* nindex, int32_t* nlist, float* w) {
+  int32_t* ids = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n * k));
+  for (int64_t t = 0; t < n * k; ++t) ids[t] = elem_id(lists[t].label);
+  int32_t* flag = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n * k));
+#pragma omp parallel for
+  for (int64_t i = 0; i < n; ++i)
+    for (int j = 0; j < k; ++j) flag[i * k + j] = find_in_list(ids, ids[i * k + j], i, k);
+  /* row sizes: own k + non-mutual reverse + clique */
+  int64_t* cnt = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
+  for (int64_t i = 0; i < n; ++i) {
+    cnt[i] += k;
+    for (int j = 0; j < k; ++j)
+      if (!flag[i * k + j]) cnt[ids[i * k + j]] += 1;
+  }
+  int32_t* ahrank = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+  for (int64_t i = 0; i < n; ++i) ahrank[i] = -1;
+  for (int64_t a = 0; a < h; ++a) ahrank[ah[a]] = (int32_t)a;
+  for (int64_t i = 0; i < n; ++i)
+    if (ahrank[i] >= 0) cnt[i] += h - 1;
+  int64_t* off = (int64_t*)malloc(sizeof(int64_t) * ((size_t)n + 1));
+  off[0] = 0;
+  for (int64_t i = 0; i < n; ++i) off[i + 1] = off[i] + cnt[i];
+  row_entry* all = (row_entry*)malloc(sizeof(row_entry) * (size_t)(off[n] > 0 ? off[n] : 1));
+  int64_t* fillc = (int64_t*)calloc((size_t)n, sizeof(int64_t));
+  for (int64_t i = 0; i < n; ++i)
+    for (int j = 0; j < k; ++j) {
+      row_entry e = {ids[i * k + j], 0, 0.0f};
+      all[off[i] + fillc[i]++] = e;
+    }
+  for (int64_t i = 0; i < n; ++i)
+    for (int j = 0; j < k; ++j)
+      if (!flag[i * k + j]) {
+        int64_t nb = ids[i * k + j];
+        row_entry e = {(int32_t)i, 0, 0.0f};
+        all[off[nb] + fillc[nb]++] = e;
+      }
+  for (int64_t a = 0; a < h; ++a)
+    for (int64_t b = 0; b < h; ++b)
+      if (a != b) {
+        row_entry e = {ah[b], 1, 0.0f};
+        all[off[ah[a]] + fillc[ah[a]]++] = e;
+      }
+  /* weights: MRD = max(core_a, core_b); clique entries raised to d_ab */
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t t = off[i]; t < off[i + 1]; ++t) {
+      row_entry* e = &all[t];
+      float wv = core[i] > core[e->nb] ? core[i] : core[e->nb];
+      if (e->is_clique) {
+        float dab = or_dist_fp64(X + i * d, X + (int64_t)e->nb * d, d);
+        if (wv < dab) wv = dab;
+      }
+      e->w = wv;
+    }
+  int64_t total = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    qsort(all + off[i], (size_t)(off[i + 1] - off[i]), sizeof(row_entry), cmp_row_entry);
+    if (nindex) nindex[i] = total;
+    for (int64_t t = off[i]; t < off[i + 1]; ++t) {
+      if (upper && all[t].nb <= i) continue;
+      if (nlist) nlist[total] = all[t].nb;
+      if (w) w[total] = all[t].w;
+      total++;
+    }
+  }
+  if (nindex) nindex[n] = total;
+  free(ids);
+  free(flag);
+  free(cnt);
+  free(ahrank);
+  free(off);
+  free(all);
+  free(fillc);
+  return total;
+}
+
+typedef struct {
+  float w;
+  int64_t j;
+  int32_t i, nb;
+} kr_tuple;
+
+static int cmp_tuple(const void* a, const void* b) {
+  const kr_tuple* x = (const kr_tuple*)a;
+  const kr_tuple* y = (const kr_tuple*)b;
+  if (x->w != y->w) return x->w < y->w ? -1 : 1;
+  if (x->j != y->j) return x->j < y->j ? -1 : 1;
+  if (x->i != y->i) return x->i < y->i ? -1 : 1;
+  return x->nb < y->nb ? -1 : (x->nb > y->nb ? 1 : 0);
+}
+
+static int32_t uf_find(int32_t* parent, int32_t x) {
+  int32_t r = x;
+  while (parent[r] != r) r = parent[r];
+  while (parent[x] != r) {
+    int32_t nx = parent[x];
+    parent[x] = r;
+    x = nx;
+  }
+  return r;
+}
+
+/* cpuMST (mst.cu:34-78): tuples (w, j, i, n) for n > i, sorted, union-find;
+ * then buildMST (:291-334) emits the chosen entries in CSR order.  Returns
+ * the number of edges; *components receives the number of trees. */
+int64_t or_kruskal(const int64_t* nindex, const int32_t* nlist, const float* w, int64_t n, or_edge* out,
+                   int64_t* components) {
+  int64_t m = 0;
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t j = nindex[i]; j < nindex[i + 1]; ++j)
+      if (nlist[j] > i) m++;
+  kr_tuple* t = (kr_tuple*)malloc(sizeof(kr_tuple) * (size_t)(m > 0 ? m : 1));
+  m = 0;
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t j = nindex[i]; j < nindex[i + 1]; ++j)
+      if (nlist[j] > i) {
+        kr_tuple x = {w[j], j, (int32_t)i, nlist[j]};
+        t[m++] = x;
+      }
+  qsort(t, (size_t)m, sizeof(kr_tuple), cmp_tuple);
+  int32_t* parent = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+  for (int64_t i = 0; i < n; ++i) parent[i] = (int32_t)i;
+  int64_t E = nindex[n];
+  unsigned char* in = (unsigned char*)calloc((size_t)(E > 0 ? E : 1), 1);
+  int64_t left = n - 1;
+  for (int64_t p = 0; p < m && left > 0; ++p) {
+    int32_t a = uf_find(parent, t[p].i), b = uf_find(parent, t[p].nb);
+    if (a != b) {
+      in[t[p].j] = 1;
+      if (a > b) parent[b] = a; else parent[a] = b;
+      left--;
+    }
+  }
+  int64_t c = 0;
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t j = nindex[i]; j < nindex[i + 1]; ++j)
+      if (in[j]) {
+        out[c].from_node = (int32_t)i;
+        out[c].to_node = nlist[j];
+        out[c].weight = w[j];
+        c++;
+      }
+  int64_t comps = 0;
+  for (int64_t i = 0; i < n; ++i)
+    if (uf_find(parent, (int32_t)i) == i) comps++;
+  *components = comps;
+  free(t);
+  free(parent);
+  free(in);
+  return c;
+}
+
+/* mergeSort (merge_sort.cu:27-116): stable ascending by weight; ties keep
+ * input order. */
+void or_sort_edges(or_edge* e, int64_t m) {
+  if (m < 2) return;
+  or_edge* tmp = (or_edge*)malloc(sizeof(or_edge) * (size_t)m);
+  for (int64_t width = 1; width < m; width *= 2) {
+    for (int64_t lo = 0; lo < m; lo += 2 * width) {
+      int64_t mid = lo + width < m ? lo + width : m;
+      int64_t hi = lo + 2 * width < m ? lo + 2 * width : m;
+      int64_t a = lo, b = mid, c = lo;
+      while (a < mid && b < hi) tmp[c++] = (e[a].weight <= e[b].weight) ? e[a++] : e[b++];
+      while (a < mid) tmp[c++] = e[a++];
+      while (b < hi) tmp[c++] = e[b++];
+    }
+    memcpy(e, tmp, sizeof(or_edge) * (size_t)m);
+  }
+  free(tmp);
+}

@@ -1,0 +1,145 @@
+"""numpy front end of the C oracle (TEST INFRASTRUCTURE, never the product).
+
+Only tests/, the smoke check and the CPU-baseline leg of bench.py may import
+this module.  Each function wraps the C restatement in hdb_oracle.c; see the
+reference citations there.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle_hdb.so"
+
+NND_ELEMENT = np.dtype([("distance", "<f4"), ("label", "<i4")])
+MST_EDGE = np.dtype([("from_node", "<i4"), ("to_node", "<i4"), ("weight", "<f4")])
+
+_lib = None
+VP, I64, I32 = C.c_void_p, C.c_int64, C.c_int32
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            from .build import build
+            build(verbose=False)
+        L = C.CDLL(str(LIB))
+        L.or_xorshift64star.restype = C.c_uint64
+        L.or_xorshift64star.argtypes = [C.c_uint64]
+        L.or_d2.restype = C.c_float
+        L.or_d2.argtypes = [VP, VP, C.c_int]
+        L.or_init_random.argtypes = [VP, I64, C.c_int, VP]
+        L.or_bruteforce_fp64.argtypes = [VP, I64, C.c_int, VP, I64, C.c_int, VP, VP]
+        L.or_recall.restype = C.c_double
+        L.or_recall.argtypes = [VP, C.c_int, VP, C.c_int, I64, C.c_int]
+        L.or_hub_counts.argtypes = [VP, I64, C.c_int, VP]
+        L.or_antihubs.argtypes = [VP, VP, I64, C.c_int, VP, C.POINTER(C.c_int64)]
+        L.or_core.argtypes = [VP, VP, I64, C.c_int, C.c_int, C.c_int, VP]
+        L.or_mrd_csr.restype = C.c_int64
+        L.or_mrd_csr.argtypes = [VP, VP, VP, VP, I64, I64, C.c_int, C.c_int, C.c_int, VP, VP, VP]
+        L.or_kruskal.restype = C.c_int64
+        L.or_kruskal.argtypes = [VP, VP, VP, I64, VP, C.POINTER(C.c_int64)]
+        L.or_sort_edges.argtypes = [VP, I64]
+        L.or_dist_fp64.restype = C.c_float
+        L.or_dist_fp64.argtypes = [VP, VP, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def xorshift64star(x: int) -> int:
+    return int(lib().or_xorshift64star(C.c_uint64(x)))
+
+
+def d2(a, b) -> np.float32:
+    a, b = _c(a, np.float32), _c(b, np.float32)
+    return np.float32(lib().or_d2(a.ctypes.data, b.ctypes.data, a.shape[0]))
+
+
+def init_random(X) -> np.ndarray:
+    X = _c(X, np.float32)
+    out = np.empty((X.shape[0], 32), NND_ELEMENT)
+    lib().or_init_random(X.ctypes.data, X.shape[0], X.shape[1], out.ctypes.data)
+    return out
+
+
+def bruteforce_fp64(X, qids, k=32):
+    X = _c(X, np.float32)
+    q = _c(qids, np.int32)
+    ids = np.empty((len(q), k), np.int32)
+    dd = np.empty((len(q), k), np.float64)
+    lib().or_bruteforce_fp64(X.ctypes.data, X.shape[0], X.shape[1], q.ctypes.data, len(q), k, ids.ctypes.data,
+                             dd.ctypes.data)
+    return ids, dd
+
+
+def recall(result_ids, gt_ids, r=32) -> float:
+    a = _c(result_ids, np.int32)
+    b = _c(gt_ids, np.int32)
+    return float(lib().or_recall(a.ctypes.data, a.shape[1], b.ctypes.data, b.shape[1], a.shape[0], r))
+
+
+def hub_counts(lists) -> np.ndarray:
+    L = _c(lists, NND_ELEMENT)
+    out = np.empty(L.shape[0], np.int32)
+    lib().or_hub_counts(L.ctypes.data, L.shape[0], L.shape[1], out.ctypes.data)
+    return out
+
+
+def antihubs(lists, counts) -> np.ndarray:
+    L = _c(lists, NND_ELEMENT)
+    c = _c(counts, np.int32)
+    n = L.shape[0]
+    ids = np.empty(int(np.sqrt(n)) + 2, np.int32)
+    h = C.c_int64()
+    lib().or_antihubs(L.ctypes.data, c.ctypes.data, n, L.shape[1], ids.ctypes.data, C.byref(h))
+    return ids[: h.value]
+
+
+def core(X, lists, mpts=32) -> np.ndarray:
+    X = _c(X, np.float32)
+    L = _c(lists, NND_ELEMENT)
+    out = np.empty(X.shape[0], np.float32)
+    lib().or_core(X.ctypes.data, L.ctypes.data, X.shape[0], X.shape[1], L.shape[1], mpts, out.ctypes.data)
+    return out
+
+
+def mrd_csr(X, lists, core_d, ah, upper=False):
+    X = _c(X, np.float32)
+    L = _c(lists, NND_ELEMENT)
+    cd = _c(core_d, np.float32)
+    a = _c(ah, np.int32)
+    n = X.shape[0]
+    nindex = np.empty(n + 1, np.int64)
+    E = lib().or_mrd_csr(X.ctypes.data, L.ctypes.data, cd.ctypes.data, a.ctypes.data, len(a), n, X.shape[1],
+                         L.shape[1], int(upper), None, None, None)
+    nlist = np.empty(max(E, 1), np.int32)
+    w = np.empty(max(E, 1), np.float32)
+    lib().or_mrd_csr(X.ctypes.data, L.ctypes.data, cd.ctypes.data, a.ctypes.data, len(a), n, X.shape[1],
+                     L.shape[1], int(upper), nindex.ctypes.data, nlist.ctypes.data, w.ctypes.data)
+    return nindex, nlist[:E], w[:E]
+
+
+def kruskal(nindex, nlist, w):
+    ni = _c(nindex, np.int64)
+    nl = _c(nlist, np.int32)
+    ww = _c(w, np.float32)
+    n = len(ni) - 1
+    out = np.empty(max(n - 1, 1), MST_EDGE)
+    comps = C.c_int64()
+    m = lib().or_kruskal(ni.ctypes.data, nl.ctypes.data if len(nl) else None, ww.ctypes.data if len(ww) else None,
+                         n, out.ctypes.data, C.byref(comps))
+    return out[:m], comps.value
+
+
+def sort_edges(edges) -> np.ndarray:
+    e = _c(edges, MST_EDGE).copy()
+    lib().or_sort_edges(e.ctypes.data, len(e))
+    return e

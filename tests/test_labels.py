@@ -1,0 +1,96 @@
+"""Labels: the C++ host implementation (hdb_labels, product) against the
+pure-Python oracle (oracle/tree_oracle.py, a literal restatement of
+HDBSCAN/trees/tree.cu), and the oracle against scikit-learn's HDBSCAN tree
+code where the semantics coincide (distinct positive weights).
+
+hdb_labels is host code in libhdbscan_mi355x.so; calling it touches no GPU.
+"""
+import numpy as np
+import pytest
+import scipy.sparse.csgraph as csg
+from scipy.spatial.distance import pdist, squareform
+
+from gpu_hdbscan_amd import api
+from oracle import tree_oracle
+
+sk_tree = pytest.importorskip("sklearn.cluster._hdbscan._tree")
+
+
+def _mst_edges(X):
+    D = squareform(pdist(X.astype(np.float64)))
+    T = csg.minimum_spanning_tree(D).tocoo()
+    e = np.empty(len(T.data), api.MST_EDGE)
+    a, b = np.minimum(T.row, T.col), np.maximum(T.row, T.col)
+    e["from_node"], e["to_node"], e["weight"] = a, b, T.data.astype(np.float32)
+    return e[np.argsort(e["weight"], kind="stable")]
+
+
+def _blobs(n, centres, seed, spread=10.0, std=1.0):
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-spread, spread, (centres, 5))
+    lab = rng.integers(0, centres, n)
+    return c[lab] + std * rng.standard_normal((n, 5))
+
+
+def _same_partition(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    if not np.array_equal(a == -1, b == -1):
+        return False
+    m = {}
+    for x, y in zip(a, b):
+        if x == -1:
+            continue
+        if m.setdefault(x, y) != y:
+            return False
+    return len(set(m.values())) == len(m)
+
+
+@pytest.mark.parametrize("seed,n,centres,mpts", [(0, 300, 3, 10), (1, 500, 5, 15), (2, 400, 2, 32)])
+def test_tree_oracle_vs_sklearn(seed, n, centres, mpts):
+    e = _mst_edges(_blobs(n, centres, seed))
+    lab, nc = tree_oracle.labels([tuple(x) for x in e], n, mpts)
+    # sklearn: single-linkage tree from the same sorted MST
+    slt = np.empty(n - 1, sk_tree.HIERARCHY_dtype)
+    parent = list(range(2 * n - 1))
+    size = [1] * n + [0] * (n - 1)
+
+    def find(x):
+        while parent[x] != x:
+            parent[x] = parent[parent[x]]
+            x = parent[x]
+        return x
+
+    for i, (a, b, w) in enumerate(e.tolist()):
+        ra, rb = find(a), find(b)
+        slt[i] = (ra, rb, w, size[ra] + size[rb])
+        size[n + i] = size[ra] + size[rb]
+        parent[ra] = parent[rb] = n + i
+    sk_lab, _ = sk_tree.tree_to_labels(slt, min_cluster_size=mpts)
+    assert nc == len(set(sk_lab.tolist()) - {-1})
+    assert _same_partition(lab, sk_lab)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_hdb_labels_matches_oracle(seed):
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.integers(50, 700))
+    X = _blobs(n, int(rng.integers(1, 6)), seed, spread=8.0)
+    e = _mst_edges(X)
+    if seed % 3 == 1:
+        e["weight"] = np.round(e["weight"], 1)  # many ties
+        e = e[np.argsort(e["weight"], kind="stable")]
+    if seed % 3 == 2:
+        e["weight"][: n // 10] = 0.0  # zero weights -> lambda = INT_MAX path
+        e = e[np.argsort(e["weight"], kind="stable")]
+    mpts = int(rng.integers(2, 40))
+    ref, rnc = tree_oracle.labels([tuple(x) for x in e], n, mpts)
+    got, gnc = api.get_clusters(e, n, mpts)
+    assert gnc == rnc
+    np.testing.assert_array_equal(got, np.array(ref, np.int32))
+
+
+def test_hdb_labels_edge_cases():
+    one, nc = api.get_clusters(np.empty(0, api.MST_EDGE), 1, 5)
+    assert one.tolist() == [-1] and nc == 0
+    with pytest.raises(api.HdbError):
+        api.get_clusters(np.empty(0, api.MST_EDGE), 5, 1)  # mpts < 2

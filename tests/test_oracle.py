@@ -1,0 +1,289 @@
+"""Pins of the CPU oracle (oracle/hdb_oracle.c) against closed forms,
+independent library routines and brute force on small inputs.
+
+The reference ships no tests, golden vectors or fixtures for this path (it
+has no test directory); each oracle function is pinned here by mathematics
+chosen so that a plausible mistake (wrong index, sign, dropped term,
+transposed operand, wrong tie order) fails a check.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.csgraph as csg
+
+from gpu_hdbscan_amd import datasets
+from oracle import oracle as O
+
+MASK = (1 << 64) - 1
+
+
+def test_xorshift64star_published_form():
+    # Vigna's xorshift64*: shifts (12, 25, 27) then multiplication by
+    # 2685821657736338717; the reference adds 42 first
+    # (gpuknn/knncuda_tools.cu:126-132).  Check against an independent
+    # vectorised numpy evaluation and by inverting the map.
+    xs = np.array([0, 1, 2, 41, 42, 12345, 2**31 - 1, 2**40 + 7, 2**63 - 1], dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        y = xs + np.uint64(42)
+        y ^= y >> np.uint64(12)
+        y ^= y << np.uint64(25)
+        y ^= y >> np.uint64(27)
+        y = y * np.uint64(2685821657736338717)
+    got = np.array([O.xorshift64star(int(x)) for x in xs], dtype=np.uint64)
+    np.testing.assert_array_equal(got, y)
+    # invert: multiply by the modular inverse, undo the xorshifts
+    inv = pow(2685821657736338717, -1, 1 << 64)
+    for x, out in zip(xs.tolist(), got.tolist()):
+        v = (out * inv) & MASK
+        v ^= v >> 27
+        v ^= v >> 54
+        t = v
+        t = v ^ ((t << 25) & MASK)
+        t = v ^ ((t << 25) & MASK)
+        t = v ^ ((t << 25) & MASK)
+        v = t
+        t = v ^ (v >> 12)
+        t = v ^ (t >> 12)
+        t = v ^ (t >> 12)
+        t = v ^ (t >> 12)
+        t = v ^ (t >> 12)
+        v = t
+        assert (v - 42) & MASK == x
+
+
+def test_d2_exact_properties():
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal(12).astype(np.float32)
+    b = rng.standard_normal(12).astype(np.float32)
+    assert O.d2(a, a) == 0.0
+    assert O.d2(a, b) == O.d2(b, a)  # (a-b)^2 == (b-a)^2 bit for bit
+    ref = np.sum((a.astype(np.float64) - b.astype(np.float64)) ** 2)
+    assert abs(float(O.d2(a, b)) - ref) <= 1e-6 * ref
+    # a single non-zero coordinate is exact
+    c = np.zeros(12, np.float32)
+    c[5] = 3.0
+    assert O.d2(c, np.zeros(12, np.float32)) == 9.0
+
+
+def test_init_random_rule():
+    X = datasets.tiny_blobs(500, 12, 4, seed=3)
+    L = O.init_random(X)
+    n = len(X)
+    for i in (0, 1, 17, 499):
+        labs = []
+        for j in range(32):
+            v = O.xorshift64star(i * 32 + j) % n
+            if v == i:
+                v = (v + 1) % n
+            labs.append(v)
+        got = L[i]["label"]
+        assert sorted(got.tolist()) == sorted(labs)  # same multiset (duplicates kept)
+        dd = np.array([O.d2(X[i], X[v]) for v in got], np.float32)
+        np.testing.assert_array_equal(L[i]["distance"], dd)
+        keys = [(float(x), int(y)) for x, y in zip(L[i]["distance"], got)]
+        assert keys == sorted(keys)
+    assert not np.any(L["label"] == np.arange(n)[:, None])
+
+
+def test_bruteforce_fp64_vs_argsort():
+    X = datasets.tiny_blobs(800, 12, 5, seed=4)
+    q = np.array([0, 5, 77, 799], np.int32)
+    ids, d2 = O.bruteforce_fp64(X, q, k=32)
+    Xd = X.astype(np.float64)
+    for r, qq in enumerate(q):
+        dist = ((Xd - Xd[qq]) ** 2).sum(1)
+        dist[qq] = np.inf
+        order = np.lexsort((np.arange(len(X)), dist))[:32]
+        np.testing.assert_array_equal(ids[r], order)
+        np.testing.assert_allclose(d2[r], dist[order], rtol=1e-12)
+
+
+def test_recall_closed_form():
+    a = np.arange(64, dtype=np.int32).reshape(2, 32)
+    assert O.recall(a, a) == 1.0
+    assert O.recall(a, a + 1000) == 0.0
+    b = a.copy()
+    b[:, 16:] += 1000
+    assert O.recall(a, b) == 0.5
+    assert O.recall(a[:, ::-1].copy(), a) == 1.0  # set semantics
+
+
+def _random_lists(n, seed):
+    rng = np.random.default_rng(seed)
+    L = np.empty((n, 32), O.NND_ELEMENT)
+    for i in range(n):
+        ids = rng.choice(np.delete(np.arange(n), i), 32, replace=False)
+        L[i]["label"] = ids
+        L[i]["distance"] = np.sort(rng.random(32).astype(np.float32))
+    return L
+
+
+def test_hub_counts_bincount():
+    L = _random_lists(300, 1)
+    np.testing.assert_array_equal(O.hub_counts(L), np.bincount(L["label"].ravel(), minlength=300))
+    # old-flagged labels count the same vertex
+    L2 = L.copy()
+    L2["label"][:, ::2] = -L2["label"][:, ::2] - 1
+    np.testing.assert_array_equal(O.hub_counts(L2), O.hub_counts(L))
+
+
+def test_antihubs_hand_case():
+    # n = 9 -> h = 3.  in-degrees chosen so that the threshold degree (the
+    # 3rd smallest) is tied between vertices 2, 5, 7 and the score (sum of
+    # the in-degrees of the vertex's own list) decides.
+    n = 9
+    L = np.empty((n, 32), O.NND_ELEMENT)
+    L["distance"] = 1.0
+    # every list points at 8 except the tie vertices, whose lists point at
+    # vertices with chosen degrees
+    base = np.full(32, 8)
+    for i in range(n):
+        L[i]["label"] = base if i != 8 else np.full(32, 0)
+    counts = np.array([32, 4, 1, 50, 60, 1, 70, 1, 0], np.int32)
+    L[2]["label"] = np.full(32, 3)   # score 32*50
+    L[5]["label"] = np.full(32, 1)   # score 32*4  (lowest)
+    L[7]["label"] = np.full(32, 0)   # score 32*32
+    ah = O.antihubs(L, counts)
+    # below threshold: vertex 8 (degree 0); ties at degree 1 need 2 more:
+    # lowest scores 5 (128) then 7 (1024)
+    np.testing.assert_array_equal(ah, [5, 7, 8])
+    counts2 = counts.copy()
+    counts2[1] = 60  # score of 5 becomes 1920 -> ties now pick 7 then 2
+    np.testing.assert_array_equal(O.antihubs(L, counts2), [2, 7, 8])
+
+
+def test_antihubs_independent_rule():
+    L = _random_lists(400, 2)
+    c = O.hub_counts(L)
+    ah = O.antihubs(L, c)
+    h = int(np.floor(np.sqrt(400)))
+    assert len(ah) == h and np.all(np.diff(ah) > 0)
+    score = c[L["label"]].sum(1)
+    order = np.lexsort((np.arange(400), score, c))  # by degree, then score, then id
+    thr = c[order[h - 1]]
+    below = np.flatnonzero(c < thr)
+    tied = np.flatnonzero(c == thr)
+    tied = tied[np.lexsort((tied, score[tied]))][: h - len(below)]
+    np.testing.assert_array_equal(ah, np.sort(np.concatenate([below, tied])))
+
+
+def test_core_matches_float64():
+    X = datasets.tiny_blobs(300, 12, 3, seed=5)
+    L = _random_lists(300, 6)
+    c = O.core(X, L, mpts=32)
+    j = L["label"][:, 31]
+    ref = np.sqrt(((X.astype(np.float64) - X[j].astype(np.float64)) ** 2).sum(1))
+    np.testing.assert_allclose(c, ref, rtol=1e-7)
+    c5 = O.core(X, L, mpts=5)
+    j5 = L["label"][:, 4]
+    np.testing.assert_allclose(c5, np.sqrt(((X.astype(np.float64) - X[j5]) ** 2).sum(1)), rtol=1e-7)
+
+
+def test_mrd_csr_sets_and_weights():
+    X = datasets.tiny_blobs(200, 12, 3, seed=7)
+    L = _random_lists(200, 8)
+    c = O.core(X, L)
+    ah = O.antihubs(L, O.hub_counts(L))
+    ni, nl, w = O.mrd_csr(X, L, c, ah)
+    n = 200
+    out_sets = [set(L[i]["label"].tolist()) for i in range(n)]
+    ahs = set(ah.tolist())
+    E = 0
+    for i in range(n):
+        row = nl[ni[i]:ni[i + 1]].tolist()
+        rev = {j for j in range(n) if i in out_sets[j]}
+        union = out_sets[i] | rev
+        clique = (ahs - {i}) if i in ahs else set()
+        assert sorted(row) == sorted(list(union) + list(clique))
+        assert row == sorted(row)
+        for pos in range(ni[i], ni[i + 1]):
+            j = nl[pos]
+            mrd = max(c[i], c[j])
+            is_dup = pos + 1 < ni[i + 1] and nl[pos + 1] == j
+            is_clique_entry = j in clique and (j not in union or (pos > ni[i] and nl[pos - 1] == j))
+            if is_clique_entry:
+                dab = np.float32(np.sqrt(((X[i].astype(np.float64) - X[j]) ** 2).sum()))
+                assert w[pos] == max(mrd, dab)
+            else:
+                assert w[pos] == mrd, (i, j, is_dup)
+        E += len(union) + len(clique)
+    assert E == len(nl) == ni[-1]
+    # upper keeps exactly the nb > row entries
+    niu, nlu, wu = O.mrd_csr(X, L, c, ah, upper=True)
+    sel = np.concatenate([np.flatnonzero(nl[ni[i]:ni[i + 1]] > i) + ni[i] for i in range(n)])
+    np.testing.assert_array_equal(nlu, nl[sel])
+    np.testing.assert_array_equal(wu, w[sel])
+
+
+def _dedup_min(ni, nl, w):
+    n = len(ni) - 1
+    rows = np.repeat(np.arange(n), np.diff(ni))
+    A = sp.coo_matrix((w.astype(np.float64), (rows, nl)), shape=(n, n)).tocsr()
+    A.sum_duplicates = None  # not used: build the min explicitly below
+    best = {}
+    for r, cc, ww in zip(rows.tolist(), nl.tolist(), w.tolist()):
+        key = (min(r, cc), max(r, cc))
+        best[key] = min(best.get(key, np.inf), ww)
+    k = np.array(list(best.keys()))
+    v = np.array(list(best.values()))
+    return sp.csr_matrix((v, (k[:, 0], k[:, 1])), shape=(n, n))
+
+
+def test_kruskal_vs_scipy_total_weight():
+    rng = np.random.default_rng(9)
+    n = 400
+    # random connected graph with distinct weights: a path plus random chords
+    pairs = {(i, i + 1) for i in range(n - 1)}
+    while len(pairs) < 3000:
+        a, b = sorted(rng.choice(n, 2, replace=False).tolist())
+        pairs.add((a, b))
+    pairs = sorted(pairs)
+    wts = rng.permutation(len(pairs)).astype(np.float32) + 1.0
+    rows = [[] for _ in range(n)]
+    for (a, b), ww in zip(pairs, wts):
+        rows[a].append((b, ww))
+        rows[b].append((a, ww))
+    ni = np.zeros(n + 1, np.int64)
+    nl, w = [], []
+    for i in range(n):
+        for b, ww in sorted(rows[i]):
+            nl.append(b)
+            w.append(ww)
+        ni[i + 1] = len(nl)
+    nl, w = np.array(nl, np.int32), np.array(w, np.float32)
+    e, comps = O.kruskal(ni, nl, w)
+    assert comps == 1 and len(e) == n - 1
+    T = csg.minimum_spanning_tree(_dedup_min(ni, nl, w))
+    assert abs(T.sum() - e["weight"].astype(np.float64).sum()) < 1e-6
+    assert np.all(e["from_node"] < e["to_node"])
+
+
+def test_kruskal_tie_break_min_id_max_id():
+    # 4-cycle 0-1-2-3-0 with all weights equal: the order (w, CSR position)
+    # over rows sorted by id takes (0,1), (0,3), (1,2) and drops (2,3).
+    ni = np.array([0, 2, 4, 6, 8], np.int64)
+    nl = np.array([1, 3, 0, 2, 1, 3, 0, 2], np.int32)
+    w = np.ones(8, np.float32)
+    e, comps = O.kruskal(ni, nl, w)
+    assert comps == 1
+    assert [(int(a), int(b)) for a, b, _ in e] == [(0, 1), (0, 3), (1, 2)]
+    # a lighter (2,3) edge changes the tree
+    w2 = w.copy()
+    w2[[5, 7]] = 0.5
+    e2, _ = O.kruskal(ni, nl, w2)
+    assert sorted((int(a), int(b)) for a, b, _ in e2) == [(0, 1), (0, 3), (2, 3)]
+    # two components
+    ni3 = np.array([0, 1, 2, 3, 4], np.int64)
+    nl3 = np.array([1, 0, 3, 2], np.int32)
+    e3, c3 = O.kruskal(ni3, nl3, np.ones(4, np.float32))
+    assert c3 == 2 and len(e3) == 2
+
+
+def test_sort_edges_stable():
+    rng = np.random.default_rng(10)
+    e = np.empty(5000, O.MST_EDGE)
+    e["from_node"] = np.arange(5000)
+    e["to_node"] = np.arange(5000) + 1
+    e["weight"] = rng.integers(0, 50, 5000).astype(np.float32) / 7
+    s = O.sort_edges(e)
+    np.testing.assert_array_equal(s, e[np.argsort(e["weight"], kind="stable")])
