@@ -10,7 +10,7 @@ from pathlib import Path
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().parent / "libhdbscan_mi355x.so"
+LIB_PATH = Path(os.environ.get("HDB_LIB_PATH") or Path(__file__).resolve().parent / "libhdbscan_mi355x.so")
 
 HDB_OK, HDB_EINVAL, HDB_ENOMEM, HDB_EDEVICE, HDB_EDISCONNECTED, HDB_EUNSUPPORTED = range(6)
 STATUS_NAMES = {0: "HDB_OK", 1: "HDB_EINVAL", 2: "HDB_ENOMEM", 3: "HDB_EDEVICE", 4: "HDB_EDISCONNECTED",
@@ -33,7 +33,8 @@ class HdbParams(C.Structure):
         ("distance_mode", C.c_int32),
         ("rerank_keep", C.c_int32),
         ("bucket_cap", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("join_keep", C.c_int32),
+        ("reserved", C.c_int32 * 6),
     ]
 
 
@@ -68,7 +69,7 @@ PROTOTYPES = {
     "hdb_timer_reset": (C.c_int, [VP]),
     "hdb_timer_read": (C.c_int, [VP, C.POINTER(C.c_char_p), C.POINTER(C.c_double), PI64, C.c_int,
                                  C.POINTER(C.c_int)]),
-    "hdb_nnd_stats": (C.c_int, [VP, PI64, PI64, PI64]),
+    "hdb_nnd_stats": (C.c_int, [VP, PI64, PI64, PI64, PI64]),
     "hdb_nndescent": (C.c_int, [VP, VP, I64, I32, C.POINTER(HdbParams), VP]),
     "hdb_nndescent_from": (C.c_int, [VP, VP, I64, I32, C.POINTER(HdbParams), I32, VP]),
     "hdb_knn_merge": (C.c_int, [VP, VP, I64, I64, I32, VP, VP, C.POINTER(HdbParams), VP]),

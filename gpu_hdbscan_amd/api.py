@@ -137,9 +137,9 @@ class Context:
         return {names[i].decode(): (ms[i], cnt[i]) for i in range(min(n.value, cap))}
 
     def nnd_stats(self) -> dict:
-        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
-        self._check(self.lib.hdb_nnd_stats(self.handle, C.byref(a), C.byref(b), C.byref(c)))
-        return {"proposals": a.value, "bucket_drops": b.value, "reverse_drops": c.value}
+        a, b, c, e = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        self._check(self.lib.hdb_nnd_stats(self.handle, C.byref(a), C.byref(b), C.byref(c), C.byref(e)))
+        return {"proposals": a.value, "bucket_drops": b.value, "reverse_drops": c.value, "candidates": e.value}
 
 
 def _as_points(ctx: Context, vectors) -> tuple[DeviceArray, int, int]:
@@ -147,10 +147,6 @@ def _as_points(ctx: Context, vectors) -> tuple[DeviceArray, int, int]:
     if len(X.shape) != 2:
         raise ValueError("vectors must be n x d")
     return X, X.shape[0], X.shape[1]
-
-
-def _out(ctx, like, dev_out, shape, dtype):
-    return ctx.empty(shape, dtype)
 
 
 def _ret(arr: DeviceArray, keep_on_device: bool):

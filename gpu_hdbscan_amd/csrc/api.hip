@@ -82,7 +82,8 @@ void hdb_params_default(hdb_params* p) {
   p->mpts = 32;
   p->distance_mode = 0;
   p->rerank_keep = 48;
-  p->bucket_cap = 96;
+  p->bucket_cap = 160;
+  p->join_keep = 4;
 }
 
 int hdb_ctx_create(hdb_ctx** out, int device) {
@@ -219,8 +220,10 @@ int hdb_timer_read(hdb_ctx* ctx, const char** names, double* ms, int64_t* launch
   return HDB_OK;
 }
 
-int hdb_nnd_stats(hdb_ctx* ctx, int64_t* proposals, int64_t* bucket_drops, int64_t* reverse_drops) {
+int hdb_nnd_stats(hdb_ctx* ctx, int64_t* proposals, int64_t* bucket_drops, int64_t* reverse_drops,
+                  int64_t* candidates) {
   HDB_CALL(check_ctx(ctx));
+  if (candidates) *candidates = ctx->stats_host[3];
   if (proposals) *proposals = ctx->stats_host[0];
   if (bucket_drops) *bucket_drops = ctx->stats_host[1];
   if (reverse_drops) *reverse_drops = ctx->stats_host[2];

@@ -18,7 +18,6 @@ namespace hdb {
 constexpr int kK = 32;                       // NEIGHB_NUM_PER_LIST
 constexpr uint32_t kInvalidId = 0x7FFFFFFFu;  // empty list slot label
 constexpr uint64_t kKeyMax = ~0ull;
-constexpr int kMaxBucketCap = 96;             // 128-wide wave sort minus the list
 constexpr int kWave = 64;
 
 // Scratch arena slots owned by a context (grown on demand, never shrunk).
@@ -33,6 +32,7 @@ enum Slot : int {
   kSlotSortKeys, kSlotSortVals, kSlotSortHist, kSlotSortAux,          // radix sort
   kSlotPipeNindex, kSlotPipeNlist, kSlotPipeW, kSlotPipeAh, kSlotPipeSrc,  // pipeline
   kSlotBfTmp, kSlotMisc0, kSlotMisc1, kSlotMisc2,
+  kSlotPropTgt, kSlotPropSE, kSlotRevSE, kSlotPart,                     // sort-routed NN-Descent
   kSlotCount
 };
 
@@ -260,6 +260,12 @@ int fill_u8(hdb_ctx* ctx, uint8_t* p, int64_t n, uint8_t v);
 int scan_i32_to_i64(hdb_ctx* ctx, const int32_t* in, int64_t n, int64_t* out);
 // Stable LSD radix sort.  Keys u32 (bits [bit0, bit1)) with u64 payload.
 int radix_sort_u32_u64(hdb_ctx* ctx, uint32_t* keys, uint64_t* vals, int64_t n, int bit0, int bit1);
+int radix_sort_u32_u32(hdb_ctx* ctx, uint32_t* keys, uint32_t* vals, int64_t n, int bit0, int bit1);
+// seg_start/seg_end[k] = [first, last+1) position of key k in sorted keys (k < nseg);
+// untouched keys keep the fill value 0.
+int segment_bounds(hdb_ctx* ctx, const uint32_t* sorted_keys, int64_t m, int64_t nseg, int32_t* seg_start,
+                   int32_t* seg_end);
+int bits_for(int64_t max_key);
 // Keys-only u64 radix sort over bits [bit0, bit1).
 int radix_sort_u64(hdb_ctx* ctx, uint64_t* keys, int64_t n, int bit0, int bit1);
 int read_i64(hdb_ctx* ctx, const int64_t* p, int64_t* out);

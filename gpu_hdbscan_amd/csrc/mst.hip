@@ -367,7 +367,7 @@ int sort_edges(hdb_ctx* ctx, hdb_mst_edge* edges, int64_t m) {
   Stage st(ctx, "sort_edges");
   uint32_t* keys;
   uint64_t* vals;
-  HDB_CALL(scratch_t(ctx, kSlotSortAux, m, &keys));
+  HDB_CALL(scratch_t(ctx, kSlotMstSrc, m, &keys));
   HDB_CALL(scratch_t(ctx, kSlotMstPos, m, &vals));
   hipLaunchKernelGGL(k_edges_split, dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream, edges, m, keys, vals);
   HDB_LAUNCHED();

@@ -1,23 +1,31 @@
 // NN-Descent kNN-graph construction for gfx950.
 //
 // Layout in HBM (n points, K = 32):
-//   lists   n x 32 hdb_nnd_element {float d2, int label}, each list sorted by
-//           the key (d2 bits, id, old-before-new); label < 0 marks "old"
-//           (tools/nndescent_element.cuh:13-18).
-//   fwd_*   n x 16 int32 forward samples, rev_* n x 16 reverse samples,
-//           counts4 = {fwd_new_n, fwd_old_n, rev_new_cnt, rev_old_cnt} x n.
-//   bucket  n x cap u64 insert proposals (key with new flag), bucket_cnt n.
-//   worst   n f32: d2 of entry 31 of each list (insert prefilter).
+//   lists     n x 32 hdb_nnd_element {float d2, int label}, each list sorted
+//             by the key (d2 bits, id, old-before-new); label < 0 marks "old"
+//             (tools/nndescent_element.cuh:13-18).
+//   fwd_*     n x 16 int32 forward samples (+ counts n).
+//   rev pairs n x 32 (key = 2q + old, value = p) for every forward sample
+//             q of p, radix-sorted by key; rev_se = [start, end) per key.
+//   proposals (target u32, key u64) compacted per workgroup, radix-sorted by
+//             target; prop_se = [start, end) per target.
+//   worst     n f32: d2 of entry 31 of each list (proposal prefilter).
 //
-// One iteration (gpuknn/nndescent.cu:2133-2166) is three launches:
-//   k_nnd_sample  PrepareGraph + PrepareReverseGraph (:44-98, :293-346)
-//   k_nnd_join    ShrinkGraph + NewNeighborsCompareKernel +
-//                 TiledNewOldNeighborsCompareKernel (:405-510, :1030-1326):
-//                 one wave64 per point; each candidate keeps its single
-//                 nearest partner (NEIGHB_CACHE_NUM = 1) and proposes it.
-//   k_nnd_update  MarkAllToOld + InsertToGlobalGraph (:1398-1403, :969-1028):
-//                 one wave per list merges the list with its proposals by a
-//                 128-key bitonic sort; no locks.
+// One iteration (gpuknn/nndescent.cu:2133-2166):
+//   k_nnd_sample   PrepareGraph (:44-98): forward samples, reverse pairs
+//   radix sort     PrepareReverseGraph (:293-346) without global atomics;
+//                  each reverse list keeps S of its pairs, evenly spaced
+//                  in source order (the reference keeps the first S arrivals)
+//   k_nnd_join     ShrinkGraph + NewNeighborsCompareKernel +
+//                  TiledNewOldNeighborsCompareKernel (:405-510, :1030-1326):
+//                  one wave64 per point; candidates propose their nearest
+//                  partners to their own lists
+//   radix sort     proposals grouped by target list
+//   k_nnd_update   MarkAllToOld + InsertToGlobalGraph (:1398-1403, :969-1028):
+//                  one wave per list merges the list with its proposals in
+//                  chunks of 96 through a 128-key bitonic sort; no locks, and
+//                  the result is the first 32 distinct keys of the union
+//                  whatever the proposal order.
 #include <algorithm>
 
 #include "hdb_internal.hpp"
@@ -25,20 +33,26 @@
 namespace hdb {
 
 constexpr int kSMax = 16;  // sample list stride (SAMPLE_NUM <= 16)
+constexpr int kJoinWaves = 4;
 
 struct NndBufs {
   int32_t* fwd_new;
   int32_t* fwd_old;
-  int32_t* rev_new;
-  int32_t* rev_old;
   int32_t* fwd_new_n;
   int32_t* fwd_old_n;
-  int32_t* rev_new_cnt;
-  int32_t* rev_old_cnt;
-  uint64_t* bucket;
-  int32_t* bucket_cnt;
+  uint32_t* rev_key;
+  uint32_t* rev_val;
+  int32_t* rev_s;  // 2n
+  int32_t* rev_e;  // 2n
+  uint32_t* prop_tgt;
+  uint64_t* prop_key;
+  int32_t* prop_s;  // n
+  int32_t* prop_e;  // n
+  unsigned long long* prop_cnt;
+  int64_t prop_cap;
   float* worst;
-  unsigned long long* stats;  // [0] proposals, [1] bucket drops, [2] reverse drops
+  unsigned long long* part;  // per join block: [0] proposals, [1] capacity drops, [2] reverse drops, [3] candidates
+  int join_keep;
 };
 
 __device__ __forceinline__ hdb_nnd_element key_to_elem(uint64_t key) {
@@ -57,7 +71,6 @@ __device__ __forceinline__ hdb_nnd_element key_to_elem(uint64_t key) {
 __device__ __forceinline__ float point_d2(const float* __restrict__ X, int d, uint32_t a, uint32_t b) {
   return d2_any(X + static_cast<int64_t>(a) * d, X + static_cast<int64_t>(b) * d, d);
 }
-
 // ------------------------------------------------------------------ init --
 // InitKNNGraphIndexKernel (nndescent.cu:1673-1692) with the xorshift64* index
 // path selected at :2118, then InitKNNGraphDistanceKernel (:1694-1726) and a
@@ -81,6 +94,21 @@ __global__ __launch_bounds__(256) void k_nnd_init_random(const float* __restrict
     lists[p * kK + j] = key_to_elem(key);
     if (j == kK - 1) worst[p] = key_d2(key);
   }
+}
+
+// n <= K + 1: list p holds every other point (lane j takes j, or j + 1 past
+// p), sorted; the remaining slots are empty.
+__global__ __launch_bounds__(256) void k_nnd_exact_small(const float* __restrict__ X, int64_t n, int d,
+                                                         hdb_nnd_element* __restrict__ lists) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t p = t >> 5;
+  const int j = static_cast<int>(t & 31);
+  uint64_t key = kKeyMax;
+  const int64_t q = j < p ? j : j + 1;
+  if (p < n && q < n) key = make_key(point_d2(X, d, static_cast<uint32_t>(p), static_cast<uint32_t>(q)),
+                                     static_cast<uint32_t>(q), 1u);
+  key = bitonic_group<32>(key);
+  if (p < n) lists[p * kK + j] = key_to_elem(key);
 }
 
 // Recompute the distances of caller-supplied lists, sort them and mark
@@ -162,9 +190,9 @@ __global__ __launch_bounds__(256) void k_nnd_init_merge(const float* __restrict_
 // Forward samples: the first S new and the first S old entries of the list
 // in list order (the reference keeps the first S that win an LDS atomic);
 // in merge mode old entries are kept only from the list's own half
-// (PrepareGraphForMerge :100-166).  Each forward sample q appends p to q's
-// reverse list through a counter, dropping appends beyond S
-// (PrepareReverseGraph :293-346).
+// (PrepareGraphForMerge :100-166).  Every forward sample q of p becomes the
+// reverse pair (2q + old, p) at slot p*32 + (16*old + rank); unused slots
+// get the key 2n, which sorts after every real key.
 template <bool MERGE>
 __global__ __launch_bounds__(256) void k_nnd_sample(const hdb_nnd_element* __restrict__ lists, int64_t n, int S,
                                                     int64_t split, NndBufs b) {
@@ -177,42 +205,43 @@ __global__ __launch_bounds__(256) void k_nnd_sample(const hdb_nnd_element* __res
     const int32_t lab = lists[p * kK + j].label;
     id = label_id(lab);
     if (id != kInvalidId && static_cast<int64_t>(id) < n) {
-      if (lab >= 0) {
-        take_new = true;
-      } else {
-        take_old = !MERGE || ((static_cast<int64_t>(id) < split) == (p < split));
-      }
+      if (lab >= 0) take_new = true;
+      else take_old = !MERGE || ((static_cast<int64_t>(id) < split) == (p < split));
     }
   }
   const unsigned sh = threadIdx.x & 32u;
   const uint32_t mnew = static_cast<uint32_t>(__ballot(take_new) >> sh);
   const uint32_t mold = static_cast<uint32_t>(__ballot(take_old) >> sh);
+  if (p >= n) return;  // 32-lane halves are uniform in p
   const uint32_t lt = (1u << j) - 1u;
-  int drops = 0;
+  const int nn = min(__popc(mnew), S), no = min(__popc(mold), S);
+  uint32_t* rk = b.rev_key + p * 32;
+  uint32_t* rv = b.rev_val + p * 32;
+  const uint32_t sentinel = static_cast<uint32_t>(2 * n);
   if (take_new) {
     const int r = __popc(mnew & lt);
     if (r < S) {
       b.fwd_new[p * kSMax + r] = static_cast<int32_t>(id);
-      const int slot = atomicAdd(&b.rev_new_cnt[id], 1);
-      if (slot < S) b.rev_new[static_cast<int64_t>(id) * kSMax + slot] = static_cast<int32_t>(p);
-      else drops++;
+      rk[r] = 2u * id;
+      rv[r] = static_cast<uint32_t>(p);
     }
   }
   if (take_old) {
     const int r = __popc(mold & lt);
     if (r < S) {
       b.fwd_old[p * kSMax + r] = static_cast<int32_t>(id);
-      const int slot = atomicAdd(&b.rev_old_cnt[id], 1);
-      if (slot < S) b.rev_old[static_cast<int64_t>(id) * kSMax + slot] = static_cast<int32_t>(p);
-      else drops++;
+      rk[16 + r] = 2u * id + 1u;
+      rv[16 + r] = static_cast<uint32_t>(p);
     }
   }
-  if (p < n && j == 0) {
-    b.fwd_new_n[p] = min(__popc(mnew), S);
-    b.fwd_old_n[p] = min(__popc(mold), S);
+  if ((j < 16 && static_cast<int>(j) >= nn) || (j >= 16 && static_cast<int>(j) - 16 >= no)) {
+    rk[j] = sentinel;
+    rv[j] = 0u;
   }
-  const uint64_t dm = __ballot(drops > 0);
-  if (__lane_id() == 0 && dm) atomicAdd(&b.stats[2], static_cast<unsigned long long>(__popcll(dm)));
+  if (j == 0) {
+    b.fwd_new_n[p] = nn;
+    b.fwd_old_n[p] = no;
+  }
 }
 
 // ------------------------------------------------------------ local join --
@@ -220,18 +249,23 @@ __global__ __launch_bounds__(256) void k_nnd_sample(const hdb_nnd_element* __res
 // sort/dedup(fwd_old + rev_old), new -= old (ShrinkGraph :405-510).
 // Candidate c (lane c) computes the squared distance to each partner -- the
 // other new candidates and the old ones if c is new (NEWNEW), only the old
-// ones if c is new in merge mode, the new ones if c is old -- keeps the
-// nearest (GetMinElement/2/3 :852-961) and proposes it to c's list unless it
-// cannot beat c's current 32nd entry.
-template <int D, int WPB, bool NEWNEW>
-__global__ __launch_bounds__(WPB * 64) void k_nnd_join(const float* __restrict__ X, int64_t n, int d, int S,
-                                                       NndBufs b, int cap) {
+// ones if c is new in merge mode, the new ones if c is old -- and proposes
+// its nearest partners (GetMinElement/2/3 :852-961) to its own list unless
+// they cannot beat the list's current 32nd entry.  Proposals of the block
+// are compacted and appended with one atomic per block.
+template <int D, bool NEWNEW, int KEEP>
+__global__ __launch_bounds__(kJoinWaves * 64) void k_nnd_join(const float* __restrict__ X, int64_t n, int d, int S,
+                                                              NndBufs b) {
+  constexpr int WPB = kJoinWaves;
   constexpr bool kOwnReg = (D > 0 && D <= 32);
   constexpr int kStride = kOwnReg ? ((D + 3) & ~3) : 0;
+  constexpr int kSlots = KEEP == 0 ? 2 : KEEP;
   extern __shared__ float lds_dyn[];
   __shared__ uint32_t s_new[WPB][64];
   __shared__ uint32_t s_old[WPB][64];
   __shared__ uint32_t s_cand[WPB][64];
+  __shared__ int32_t s_wcnt[WPB];
+  __shared__ unsigned long long s_base;
   const unsigned lane = __lane_id();
   const unsigned w = threadIdx.x >> 6;
   const int64_t p = static_cast<int64_t>(blockIdx.x) * WPB + w;
@@ -241,15 +275,23 @@ __global__ __launch_bounds__(WPB * 64) void k_nnd_join(const float* __restrict__
   float* vec = lds_dyn + static_cast<int64_t>(w) * 64 * stride;
 
   uint32_t v = 0xFFFFFFFFu;
+  int rev_drop = 0;
   if (live) {
     const int part = lane >> 4, r = lane & 15;
-    int cnt;
-    const int32_t* src;
-    if (part == 0) { cnt = b.fwd_new_n[p]; src = b.fwd_new; }
-    else if (part == 1) { cnt = min(b.rev_new_cnt[p], S); src = b.rev_new; }
-    else if (part == 2) { cnt = b.fwd_old_n[p]; src = b.fwd_old; }
-    else { cnt = min(b.rev_old_cnt[p], S); src = b.rev_old; }
-    if (r < cnt) v = static_cast<uint32_t>(src[p * kSMax + r]);
+    if (part == 0) {
+      if (r < b.fwd_new_n[p]) v = static_cast<uint32_t>(b.fwd_new[p * kSMax + r]);
+    } else if (part == 2) {
+      if (r < b.fwd_old_n[p]) v = static_cast<uint32_t>(b.fwd_old[p * kSMax + r]);
+    } else {
+      const int64_t seg = 2 * p + (part == 3 ? 1 : 0);
+      const int s0 = b.rev_s[seg];
+      const int len = b.rev_e[seg] - s0;
+      if (r < min(len, S)) {
+        const int idx = s0 + (len <= S ? r : static_cast<int>((static_cast<int64_t>(r) * len) / S));
+        v = b.rev_val[idx];
+      }
+      if (r == 0 && len > S) rev_drop = len - S;
+    }
   }
   v = bitonic_group_u32<32>(v);
   const uint32_t prev = __shfl_up(v, 1);
@@ -271,7 +313,7 @@ __global__ __launch_bounds__(WPB * 64) void k_nnd_join(const float* __restrict__
     x = s_new[w][lane];
     int lo = 0, hi = no;
     while (lo < hi) {
-      int mid = (lo + hi) >> 1;
+      const int mid = (lo + hi) >> 1;
       if (s_old[w][mid] < x) lo = mid + 1; else hi = mid;
     }
     keep_new = !(lo < no && s_old[w][lo] == x);
@@ -318,7 +360,13 @@ __global__ __launch_bounds__(WPB * 64) void k_nnd_join(const float* __restrict__
     jb = 0;
     je = nn;
   }
-  uint64_t best = kKeyMax;
+  // KEEP == 0: the reference's two kernels -- a new candidate takes its
+  // nearest new partner (NewNeighborsCompareKernel) and its nearest old
+  // partner (TiledNewOldNeighborsCompareKernel), an old one its nearest new
+  // partner.  KEEP == T > 0: every candidate proposes its T nearest partners.
+  uint64_t top[kSlots];
+#pragma unroll
+  for (int s = 0; s < kSlots; ++s) top[s] = kKeyMax;
   for (int j = 0; j < nc; ++j) {
     const float* row = vec + j * stride;
     float acc;
@@ -333,75 +381,138 @@ __global__ __launch_bounds__(WPB * 64) void k_nnd_join(const float* __restrict__
       acc = d2_any(vec + lane * stride, row, dd);
     }
     const bool ok = active && j >= jb && j < je && j != static_cast<int>(lane);
-    const uint64_t key = make_key(acc, s_cand[w][j], 1u);
-    if (ok && key < best) best = key;
+    uint64_t key = ok ? make_key(acc, s_cand[w][j], 1u) : kKeyMax;
+    if (KEEP == 0) {
+      if (j < nn) top[0] = key < top[0] ? key : top[0];
+      else top[1] = key < top[1] ? key : top[1];
+    } else {
+#pragma unroll
+      for (int s = 0; s < kSlots; ++s) {
+        const uint64_t a = top[s];
+        const bool sw = key < a;
+        top[s] = sw ? key : a;
+        key = sw ? a : key;
+      }
+    }
   }
-  bool propose = false;
-  if (active && best != kKeyMax) propose = key_d2(best) < b.worst[my_id];
-  bool dropped = false;
-  if (propose) {
-    const int slot = atomicAdd(&b.bucket_cnt[my_id], 1);
-    if (slot < cap) b.bucket[static_cast<int64_t>(my_id) * cap + slot] = best;
-    else dropped = true;
+  const float lim = active ? b.worst[my_id] : 0.0f;
+  int cnt = 0;
+#pragma unroll
+  for (int s = 0; s < kSlots; ++s) {
+    const bool ok = active && top[s] != kKeyMax && key_d2(top[s]) < lim;
+    if (!ok) top[s] = kKeyMax;
+    cnt += ok ? 1 : 0;
   }
-  const uint64_t pm = __ballot(propose), dm = __ballot(dropped);
+  // compaction: exclusive prefix of lane counts, wave offsets, one atomic
+  int incl = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(incl, o);
+    if (lane >= static_cast<unsigned>(o)) incl += u;
+  }
+  if (lane == 63) s_wcnt[w] = incl;
+  __syncthreads();
+  int woff = 0, total = 0;
+#pragma unroll
+  for (int q = 0; q < WPB; ++q) {
+    if (q < static_cast<int>(w)) woff += s_wcnt[q];
+    total += s_wcnt[q];
+  }
+  if (threadIdx.x == 0) s_base = total ? atomicAdd(b.prop_cnt, static_cast<unsigned long long>(total)) : 0ull;
+  __syncthreads();
+  const int64_t base = static_cast<int64_t>(s_base);
+  const bool fits = base + total <= b.prop_cap;
+  if (fits) {
+    int64_t o = base + woff + incl - cnt;
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) {
+      if (top[s] != kKeyMax) {
+        b.prop_tgt[o] = my_id;
+        b.prop_key[o] = top[s];
+        ++o;
+      }
+    }
+  }
+  if (threadIdx.x == 0) {
+    unsigned long long* pp = b.part + 4 * static_cast<int64_t>(blockIdx.x);
+    pp[0] += static_cast<unsigned long long>(total);
+    if (!fits) pp[1] += static_cast<unsigned long long>(total);
+  }
+  // reverse-sample drops (lanes 16 and 48 hold them) and candidate counts
+  int rd = rev_drop;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) rd += __shfl_xor(rd, o);
+  __syncthreads();  // s_wcnt reuse
   if (lane == 0) {
-    if (pm) atomicAdd(&b.stats[0], static_cast<unsigned long long>(__popcll(pm)));
-    if (dm) atomicAdd(&b.stats[1], static_cast<unsigned long long>(__popcll(dm)));
+    s_wcnt[w] = rd;
+    s_new[w][0] = static_cast<uint32_t>(nc);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0, cand = 0;
+#pragma unroll
+    for (int q = 0; q < WPB; ++q) {
+      tot += s_wcnt[q];
+      cand += static_cast<int>(s_new[q][0]);
+    }
+    b.part[4 * static_cast<int64_t>(blockIdx.x) + 2] += static_cast<unsigned long long>(tot);
+    b.part[4 * static_cast<int64_t>(blockIdx.x) + 3] += static_cast<unsigned long long>(cand);
   }
 }
 
 // ----------------------------------------------------------- list update --
-// One wave per list: the 32 list entries (all marked old) and up to 96
-// proposals (new) are sorted as 128 keys; equal (distance, id) keys collapse
-// to the first (the old entry wins, as InsertToGlobalGraph rejects an equal
-// element); the first 32 survivors form the new list.
-template <int WPB>
-__global__ __launch_bounds__(WPB * 64) void k_nnd_update(hdb_nnd_element* __restrict__ lists, int64_t n, NndBufs b,
-                                                         int cap) {
+// One wave per list: the 32 list entries (all marked old) and the list's
+// proposals (new), 96 at a time, are sorted as 128 keys; equal (distance, id)
+// keys collapse to the first (the old entry wins, as InsertToGlobalGraph
+// rejects an equal element); the first 32 survivors carry over.
+__global__ __launch_bounds__(256) void k_nnd_update(hdb_nnd_element* __restrict__ lists, int64_t n, NndBufs b,
+                                                    int full) {
+  __shared__ uint64_t s_top[4][32];
   const unsigned lane = __lane_id();
-  const int64_t t = static_cast<int64_t>(blockIdx.x) * WPB + (threadIdx.x >> 6);
-  if (t >= n) return;  // whole wave exits together
-  const int cnt = min(b.bucket_cnt[t], cap);
-  const uint64_t* bk = b.bucket + t * cap;
-  uint64_t r0, r1;
+  const unsigned w = threadIdx.x >> 6;
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * 4 + w;
+  if (t >= n) return;  // whole wave exits together; no block barriers below
+  const int64_t ps = b.prop_s[t], pe = b.prop_e[t];
+  hdb_nnd_element* out = lists + t * kK;
+  uint64_t r0 = kKeyMax, r1;
   if (lane < 32) {
-    const hdb_nnd_element e = lists[t * kK + lane];
+    const hdb_nnd_element e = out[lane];
     const uint32_t id = label_id(e.label);
     r0 = (id == kInvalidId) ? kKeyMax : make_key(e.distance, id, 0u);
-  } else {
-    const int j = lane - 32;
-    r0 = j < cnt ? bk[j] : kKeyMax;
   }
-  {
-    const int j = 32 + lane;
-    r1 = j < cnt ? bk[j] : kKeyMax;
+  if (ps == pe && !full) {  // no proposals: mark old
+    if (lane < 32) out[lane] = key_to_elem(r0);
+    return;
   }
-  bitonic128(r0, r1);
-  // predecessor of element i = r*64 + lane in sorted order
-  const uint64_t p0 = shfl_u64(r0, (lane + 63) & 63);
-  const uint64_t q1 = shfl_u64(r1, (lane + 63) & 63);
-  const uint64_t last0 = shfl_u64(r0, 63);
-  const uint64_t p1 = lane == 0 ? last0 : q1;
-  const bool keep0 = r0 != kKeyMax && (lane == 0 || (p0 >> 1) != (r0 >> 1));
-  const bool keep1 = r1 != kKeyMax && ((p1 >> 1) != (r1 >> 1));
-  const uint64_t b0 = __ballot(keep0), b1 = __ballot(keep1);
+  // one pass at least: the first update also removes duplicate init entries
   const uint64_t lt = lanemask_lt();
-  const int pos0 = __popcll(b0 & lt);
-  const int pos1 = __popcll(b0) + __popcll(b1 & lt);
-  const int total = __popcll(b0) + __popcll(b1);
-  hdb_nnd_element* out = lists + t * kK;
-  if (keep0 && pos0 < kK) out[pos0] = key_to_elem(r0);
-  if (keep1 && pos1 < kK) out[pos1] = key_to_elem(r1);
-  if (total < kK && static_cast<int>(lane) >= total && lane < static_cast<unsigned>(kK)) out[lane] = key_to_elem(kKeyMax);
-  if (keep0 && pos0 == kK - 1) b.worst[t] = key_d2(r0);
-  if (keep1 && pos1 == kK - 1) b.worst[t] = key_d2(r1);
-  if (lane == 0) {
-    if (total < kK) b.worst[t] = __uint_as_float(0x7F800000u);
-    b.bucket_cnt[t] = 0;
-    b.rev_new_cnt[t] = 0;
-    b.rev_old_cnt[t] = 0;
-  }
+  int64_t c0 = ps;
+  do {
+    if (lane >= 32) r0 = (c0 + lane - 32 < pe) ? b.prop_key[c0 + lane - 32] : kKeyMax;
+    r1 = (c0 + 32 + lane < pe) ? b.prop_key[c0 + 32 + lane] : kKeyMax;
+    bitonic128(r0, r1);
+    const uint64_t p0 = shfl_u64(r0, (lane + 63) & 63);
+    const uint64_t q1 = shfl_u64(r1, (lane + 63) & 63);
+    const uint64_t last0 = shfl_u64(r0, 63);
+    const uint64_t p1 = lane == 0 ? last0 : q1;
+    const bool keep0 = r0 != kKeyMax && (lane == 0 || (p0 >> 1) != (r0 >> 1));
+    const bool keep1 = r1 != kKeyMax && ((p1 >> 1) != (r1 >> 1));
+    const uint64_t b0 = __ballot(keep0), b1 = __ballot(keep1);
+    const int pos0 = __popcll(b0 & lt);
+    const int pos1 = __popcll(b0) + __popcll(b1 & lt);
+    const int total = __popcll(b0) + __popcll(b1);
+    if (keep0 && pos0 < kK) s_top[w][pos0] = r0;
+    if (keep1 && pos1 < kK) s_top[w][pos1] = r1;
+    if (static_cast<int>(lane) >= total && lane < 32u) s_top[w][lane] = kKeyMax;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    r0 = lane < 32 ? s_top[w][lane] : kKeyMax;
+    __builtin_amdgcn_wave_barrier();
+    c0 += 96;
+  } while (c0 < pe);
+  if (lane < 32) out[lane] = key_to_elem(r0);
+  if (lane == 31) b.worst[t] = (r0 == kKeyMax) ? __uint_as_float(0x7F800000u) : key_d2(r0);
 }
 
 // Strip old flags so exported labels are >= 0.
@@ -411,6 +522,23 @@ __global__ void k_nnd_finalize(hdb_nnd_element* __restrict__ lists, int64_t tota
     const int32_t lab = lists[i].label;
     if (lab < 0) lists[i].label = -lab - 1;
   }
+}
+
+__global__ void k_sum_parts(const unsigned long long* __restrict__ part, int64_t blocks,
+                            unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long acc[4][256];
+  unsigned long long a[4] = {0, 0, 0, 0};
+  for (int64_t i = threadIdx.x; i < blocks; i += 256)
+    for (int q = 0; q < 4; ++q) a[q] += part[4 * i + q];
+  for (int q = 0; q < 4; ++q) acc[q][threadIdx.x] = a[q];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (static_cast<int>(threadIdx.x) < s)
+      for (int q = 0; q < 4; ++q) acc[q][threadIdx.x] += acc[q][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 4; ++q) out[q] = acc[q][0];
 }
 
 // --------------------------------------------------------- unique merge --
@@ -462,61 +590,82 @@ __global__ void k_offset_labels(const hdb_nnd_element* __restrict__ in, hdb_nnd_
 }
 
 // ------------------------------------------------------------ host side --
-static int get_bufs(hdb_ctx* ctx, int64_t n, int cap, NndBufs* b) {
+static int get_bufs(hdb_ctx* ctx, int64_t n, const hdb_params& prm, NndBufs* b) {
   HDB_CALL(scratch_t(ctx, kSlotFwdNew, n * kSMax, &b->fwd_new));
   HDB_CALL(scratch_t(ctx, kSlotFwdOld, n * kSMax, &b->fwd_old));
-  HDB_CALL(scratch_t(ctx, kSlotRevNew, n * kSMax, &b->rev_new));
-  HDB_CALL(scratch_t(ctx, kSlotRevOld, n * kSMax, &b->rev_old));
-  int32_t* c4;
-  HDB_CALL(scratch_t(ctx, kSlotCounts4, 4 * n, &c4));
-  b->fwd_new_n = c4;
-  b->fwd_old_n = c4 + n;
-  b->rev_new_cnt = c4 + 2 * n;
-  b->rev_old_cnt = c4 + 3 * n;
-  HDB_CALL(scratch_t(ctx, kSlotBucket, n * cap, &b->bucket));
-  HDB_CALL(scratch_t(ctx, kSlotBucketCnt, n, &b->bucket_cnt));
+  HDB_CALL(scratch_t(ctx, kSlotRevNew, n * 32, &b->rev_key));
+  HDB_CALL(scratch_t(ctx, kSlotRevOld, n * 32, &b->rev_val));
+  int32_t* c2;
+  HDB_CALL(scratch_t(ctx, kSlotCounts4, 2 * n, &c2));
+  b->fwd_new_n = c2;
+  b->fwd_old_n = c2 + n;
+  int32_t* rse;
+  HDB_CALL(scratch_t(ctx, kSlotRevSE, 4 * n, &rse));
+  b->rev_s = rse;
+  b->rev_e = rse + 2 * n;
+  b->prop_cap = n * static_cast<int64_t>(prm.bucket_cap);
+  HDB_CALL(scratch_t(ctx, kSlotPropTgt, b->prop_cap, &b->prop_tgt));
+  HDB_CALL(scratch_t(ctx, kSlotBucket, b->prop_cap, &b->prop_key));
+  int32_t* pse;
+  HDB_CALL(scratch_t(ctx, kSlotPropSE, 2 * n, &pse));
+  b->prop_s = pse;
+  b->prop_e = pse + n;
   HDB_CALL(scratch_t(ctx, kSlotWorst, n, &b->worst));
-  HDB_CALL(scratch_t(ctx, kSlotStats, 4, &b->stats));
-  HDB_CALL(fill_i32(ctx, c4, 4 * n, 0));
-  HDB_CALL(fill_i32(ctx, b->bucket_cnt, n, 0));
-  HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(b->stats), 4, 0));
+  unsigned long long* st;
+  HDB_CALL(scratch_t(ctx, kSlotStats, 8, &st));
+  b->prop_cnt = st;
+  const int64_t blocks = (n + kJoinWaves - 1) / kJoinWaves;
+  HDB_CALL(scratch_t(ctx, kSlotPart, 4 * blocks, &b->part));
+  HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(b->part), 4 * blocks, 0));
+  HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(st), 8, 0));
+  b->join_keep = prm.join_keep;
   return HDB_OK;
 }
 
-static int read_stats(hdb_ctx* ctx, const NndBufs& b) {
+static int read_stats(hdb_ctx* ctx, int64_t n, const NndBufs& b) {
+  const int64_t blocks = (n + kJoinWaves - 1) / kJoinWaves;
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, ctx->stream, b.part, blocks, b.prop_cnt + 4);
+  HDB_LAUNCHED();
   unsigned long long s[4];
-  HDB_HIP(hipMemcpyAsync(s, b.stats, sizeof(s), hipMemcpyDeviceToHost, ctx->stream));
+  HDB_HIP(hipMemcpyAsync(s, b.prop_cnt + 4, sizeof(s), hipMemcpyDeviceToHost, ctx->stream));
   HDB_HIP(hipStreamSynchronize(ctx->stream));
-  ctx->stats_host[0] = static_cast<int64_t>(s[0]);
-  ctx->stats_host[1] = static_cast<int64_t>(s[1]);
-  ctx->stats_host[2] = static_cast<int64_t>(s[2]);
+  for (int i = 0; i < 4; ++i) ctx->stats_host[i] = static_cast<int64_t>(s[i]);
   return HDB_OK;
 }
 
-template <int D, int WPB, bool NEWNEW>
-static int launch_join_t(hdb_ctx* ctx, const float* X, int64_t n, int d, int S, const NndBufs& b, int cap) {
+template <int D, bool NEWNEW, int KEEP>
+static int launch_join_k(hdb_ctx* ctx, const float* X, int64_t n, int d, int S, const NndBufs& b) {
   constexpr bool kOwnReg = (D > 0 && D <= 32);
   const int stride = kOwnReg ? ((D + 3) & ~3) : d + 1;
-  const size_t lds = static_cast<size_t>(WPB) * 64 * stride * sizeof(float);
-  hipLaunchKernelGGL((k_nnd_join<D, WPB, NEWNEW>), dim3(grid_for(n, WPB)), dim3(WPB * 64), lds, ctx->stream, X, n, d,
-                     S, b, cap);
+  const size_t lds = static_cast<size_t>(kJoinWaves) * 64 * stride * sizeof(float);
+  hipLaunchKernelGGL((k_nnd_join<D, NEWNEW, KEEP>), dim3(grid_for(n, kJoinWaves)), dim3(kJoinWaves * 64), lds,
+                     ctx->stream, X, n, d, S, b);
   HDB_LAUNCHED();
   return HDB_OK;
 }
 
-template <bool NEWNEW>
-static int launch_join(hdb_ctx* ctx, const float* X, int64_t n, int d, int S, const NndBufs& b, int cap) {
-  if (d == 12) return launch_join_t<12, 4, NEWNEW>(ctx, X, n, d, S, b, cap);
-  if (d == 32) return launch_join_t<32, 4, NEWNEW>(ctx, X, n, d, S, b, cap);
-  // generic path: one candidate row of d+1 floats per lane in LDS, <= 64 KiB per block
-  const size_t per_wave = static_cast<size_t>(64) * (d + 1) * sizeof(float);
-  if (4 * per_wave <= 65536) return launch_join_t<0, 4, NEWNEW>(ctx, X, n, d, S, b, cap);
-  if (2 * per_wave <= 65536) return launch_join_t<0, 2, NEWNEW>(ctx, X, n, d, S, b, cap);
-  return launch_join_t<0, 1, NEWNEW>(ctx, X, n, d, S, b, cap);
+template <int D, bool NEWNEW>
+static int launch_join_t(hdb_ctx* ctx, const float* X, int64_t n, int d, int S, const NndBufs& b) {
+  switch (b.join_keep) {
+    case 0: return launch_join_k<D, NEWNEW, 0>(ctx, X, n, d, S, b);
+    case 1: return launch_join_k<D, NEWNEW, 1>(ctx, X, n, d, S, b);
+    case 2: return launch_join_k<D, NEWNEW, 2>(ctx, X, n, d, S, b);
+    case 3: return launch_join_k<D, NEWNEW, 3>(ctx, X, n, d, S, b);
+    default: return launch_join_k<D, NEWNEW, 4>(ctx, X, n, d, S, b);
+  }
 }
 
-static int iterate(hdb_ctx* ctx, const float* X, int64_t n, int d, int S, int cap, bool merge, int64_t split,
-                   int iterations, hdb_nnd_element* lists, const NndBufs& b) {
+template <bool NEWNEW>
+static int launch_join(hdb_ctx* ctx, const float* X, int64_t n, int d, int S, const NndBufs& b) {
+  if (d == 12) return launch_join_t<12, NEWNEW>(ctx, X, n, d, S, b);
+  if (d == 32) return launch_join_t<32, NEWNEW>(ctx, X, n, d, S, b);
+  return launch_join_t<0, NEWNEW>(ctx, X, n, d, S, b);
+}
+
+static int iterate(hdb_ctx* ctx, const float* X, int64_t n, int d, int S, bool merge, int64_t split, int iterations,
+                   hdb_nnd_element* lists, NndBufs& b) {
+  const int rev_bits = bits_for(2 * n);
+  const int tgt_bits = bits_for(n - 1);
   for (int it = 0; it < iterations; ++it) {
     {
       Stage st(ctx, merge ? "nnd_sample_merge" : "nnd_sample");
@@ -529,13 +678,27 @@ static int iterate(hdb_ctx* ctx, const float* X, int64_t n, int d, int S, int ca
       HDB_LAUNCHED();
     }
     {
+      Stage st(ctx, "nnd_reverse_sort");
+      HDB_CALL(radix_sort_u32_u32(ctx, b.rev_key, b.rev_val, n * 32, 0, rev_bits));
+      HDB_CALL(segment_bounds(ctx, b.rev_key, n * 32, 2 * n, b.rev_s, b.rev_e));
+      HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(b.prop_cnt), 1, 0));
+    }
+    {
       Stage st(ctx, merge ? "nnd_join_merge" : "nnd_join");
-      if (merge) HDB_CALL(launch_join<false>(ctx, X, n, d, S, b, cap));
-      else HDB_CALL(launch_join<true>(ctx, X, n, d, S, b, cap));
+      if (merge) HDB_CALL(launch_join<false>(ctx, X, n, d, S, b));
+      else HDB_CALL(launch_join<true>(ctx, X, n, d, S, b));
+    }
+    int64_t m = 0;
+    HDB_CALL(read_i64(ctx, reinterpret_cast<const int64_t*>(b.prop_cnt), &m));
+    m = std::min(m, b.prop_cap);
+    {
+      Stage st(ctx, "nnd_proposal_sort");
+      HDB_CALL(radix_sort_u32_u64(ctx, b.prop_tgt, b.prop_key, m, 0, tgt_bits));
+      HDB_CALL(segment_bounds(ctx, b.prop_tgt, m, n, b.prop_s, b.prop_e));
     }
     {
       Stage st(ctx, merge ? "nnd_update_merge" : "nnd_update");
-      hipLaunchKernelGGL(k_nnd_update<4>, dim3(grid_for(n, 4)), dim3(256), 0, ctx->stream, lists, n, b, cap);
+      hipLaunchKernelGGL(k_nnd_update, dim3(grid_for(n, 4)), dim3(256), 0, ctx->stream, lists, n, b, it == 0 ? 1 : 0);
       HDB_LAUNCHED();
     }
   }
@@ -547,10 +710,11 @@ static int check_params(hdb_ctx* ctx, const hdb_params& prm, int64_t n, int d) {
   if (prm.sample_num < 1 || prm.sample_num > kSMax) return ctx->fail(HDB_EINVAL, "sample_num must be in [1,16]");
   if (prm.merge_sample_num < 1 || prm.merge_sample_num > kSMax)
     return ctx->fail(HDB_EINVAL, "merge_sample_num must be in [1,16]");
-  if (prm.bucket_cap < 1 || prm.bucket_cap > kMaxBucketCap) return ctx->fail(HDB_EINVAL, "bucket_cap must be in [1,96]");
+  if (prm.bucket_cap < 8 || prm.bucket_cap > 1024) return ctx->fail(HDB_EINVAL, "bucket_cap must be in [8,1024]");
   if (prm.nnd_iteration < 0 || prm.merge_iteration < 0) return ctx->fail(HDB_EINVAL, "negative iteration count");
+  if (prm.join_keep < 0 || prm.join_keep > 4) return ctx->fail(HDB_EINVAL, "join_keep must be in [0,4]");
   if (n < 2) return ctx->fail(HDB_EINVAL, "need at least 2 points");
-  if (n >= static_cast<int64_t>(kInvalidId)) return ctx->fail(HDB_EINVAL, "n too large for int32 labels");
+  if (n >= (static_cast<int64_t>(1) << 30)) return ctx->fail(HDB_EINVAL, "n must be below 2^30");
   if (d < 1) return ctx->fail(HDB_EINVAL, "d must be >= 1");
   if (d > 255) return ctx->fail(HDB_EUNSUPPORTED, "d above 255 (one LDS row per candidate)");
   return HDB_OK;
@@ -560,8 +724,16 @@ int nnd_run(hdb_ctx* ctx, const float* X, int64_t n, int d, const hdb_params& pr
             bool init_random, int iterations) {
   HDB_CALL(check_params(ctx, prm, n, d));
   if (!X || !lists) return ctx->fail(HDB_EINVAL, "null pointer");
+  if (n <= kK + 1 && iterations > 0) {
+    // every other point fits in each list: the exact graph, sorted
+    Stage st(ctx, "nnd_exact_small");
+    hipLaunchKernelGGL(k_nnd_exact_small, dim3(grid_for(n * kK, 256)), dim3(256), 0, ctx->stream, X, n, d, lists);
+    HDB_LAUNCHED();
+    ctx->stats_host[0] = ctx->stats_host[1] = ctx->stats_host[2] = ctx->stats_host[3] = 0;
+    return HDB_OK;
+  }
   NndBufs b;
-  HDB_CALL(get_bufs(ctx, n, prm.bucket_cap, &b));
+  HDB_CALL(get_bufs(ctx, n, prm, &b));
   {
     Stage st(ctx, "nnd_init");
     if (init_random)
@@ -572,13 +744,13 @@ int nnd_run(hdb_ctx* ctx, const float* X, int64_t n, int d, const hdb_params& pr
                          b.worst);
     HDB_LAUNCHED();
   }
-  HDB_CALL(iterate(ctx, X, n, d, prm.sample_num, prm.bucket_cap, false, 0, iterations, lists, b));
+  HDB_CALL(iterate(ctx, X, n, d, prm.sample_num, false, 0, iterations, lists, b));
   {
     Stage st(ctx, "nnd_finalize");
     hipLaunchKernelGGL(k_nnd_finalize, dim3(grid_for(n * kK, 256)), dim3(256), 0, ctx->stream, lists, n * kK);
     HDB_LAUNCHED();
   }
-  return read_stats(ctx, b);
+  return read_stats(ctx, n, b);
 }
 
 int nnd_merge_pair(hdb_ctx* ctx, const float* X, int64_t n1, int64_t n2, int d, const hdb_nnd_element* g1,
@@ -588,20 +760,20 @@ int nnd_merge_pair(hdb_ctx* ctx, const float* X, int64_t n1, int64_t n2, int d, 
   if (n1 < 1 || n2 < 1) return ctx->fail(HDB_EINVAL, "both graphs must be non-empty");
   if (!X || !g1 || !g2 || !out) return ctx->fail(HDB_EINVAL, "null pointer");
   NndBufs b;
-  HDB_CALL(get_bufs(ctx, n, prm.bucket_cap, &b));
+  HDB_CALL(get_bufs(ctx, n, prm, &b));
   {
     Stage st(ctx, "nnd_init_merge");
     hipLaunchKernelGGL(k_nnd_init_merge, dim3(grid_for(n * kK, 256)), dim3(256), 0, ctx->stream, X, n1, n2, d, g1, g2,
                        out, b.worst);
     HDB_LAUNCHED();
   }
-  HDB_CALL(iterate(ctx, X, n, d, prm.merge_sample_num, prm.bucket_cap, true, n1, prm.merge_iteration, out, b));
+  HDB_CALL(iterate(ctx, X, n, d, prm.merge_sample_num, true, n1, prm.merge_iteration, out, b));
   {
     Stage st(ctx, "nnd_finalize");
     hipLaunchKernelGGL(k_nnd_finalize, dim3(grid_for(n * kK, 256)), dim3(256), 0, ctx->stream, out, n * kK);
     HDB_LAUNCHED();
   }
-  return read_stats(ctx, b);
+  return read_stats(ctx, n, b);
 }
 
 int nnd_blocked(hdb_ctx* ctx, const float* X, int64_t n, int d, int blocks, const hdb_params& prm,
@@ -614,13 +786,13 @@ int nnd_blocked(hdb_ctx* ctx, const float* X, int64_t n, int d, int blocks, cons
     if (off[i + 1] - off[i] < 2) return ctx->fail(HDB_EINVAL, "block smaller than 2 points");
   hdb_nnd_element* local;
   HDB_CALL(scratch_t(ctx, kSlotBlockLocal, n * kK, &local));
-  int64_t stats[3] = {0, 0, 0};
+  int64_t stats[4] = {0, 0, 0, 0};
   // BuildEachShard (gen_large_knngraph.cu:47-92): NN-Descent per block with
   // local ids; the running lists start as the block graphs in global ids.
   for (int i = 0; i < blocks; ++i) {
     const int64_t ni = off[i + 1] - off[i];
     HDB_CALL(nnd_run(ctx, X + off[i] * d, ni, d, prm, local + off[i] * kK, true, prm.nnd_iteration));
-    for (int s = 0; s < 3; ++s) stats[s] += ctx->stats_host[s];
+    for (int s = 0; s < 4; ++s) stats[s] += ctx->stats_host[s];
     hipLaunchKernelGGL(k_offset_labels, dim3(grid_for(ni * kK, 256)), dim3(256), 0, ctx->stream, local + off[i] * kK,
                        out + off[i] * kK, ni * kK, off[i]);
     HDB_LAUNCHED();
@@ -645,7 +817,7 @@ int nnd_blocked(hdb_ctx* ctx, const float* X, int64_t n, int d, int blocks, cons
         xpair = xp;
       }
       HDB_CALL(nnd_merge_pair(ctx, xpair, ni, nj, d, local + off[i] * kK, local + off[j] * kK, prm, pair));
-      for (int s = 0; s < 3; ++s) stats[s] += ctx->stats_host[s];
+      for (int s = 0; s < 4; ++s) stats[s] += ctx->stats_host[s];
       Stage st(ctx, "nnd_unique_merge");
       hipLaunchKernelGGL(k_unique_merge, dim3(grid_for(ni, 4)), dim3(256), 0, ctx->stream, out + off[i] * kK, pair,
                          ni, ni, static_cast<int64_t>(0), off[i], off[j]);
@@ -655,7 +827,7 @@ int nnd_blocked(hdb_ctx* ctx, const float* X, int64_t n, int d, int blocks, cons
       HDB_LAUNCHED();
     }
   }
-  for (int s = 0; s < 3; ++s) ctx->stats_host[s] = stats[s];
+  for (int s = 0; s < 4; ++s) ctx->stats_host[s] = stats[s];
   return HDB_OK;
 }
 

@@ -169,46 +169,17 @@ __global__ void k_rs_hist(const K* __restrict__ keys, int64_t n, int shift, int6
   for (int i = threadIdx.x; i < kRsBins; i += kRsBlock) hist[static_cast<int64_t>(i) * ntiles + blockIdx.x] = h[i];
 }
 
-// Exclusive scan of the digit-major histogram (kRsBins * ntiles int32) in
-// place, by one block.
-__global__ void k_rs_scan(int32_t* __restrict__ hist, int64_t total) {
-  __shared__ int64_t carry_s;
-  if (threadIdx.x == 0) carry_s = 0;
-  __syncthreads();
-  constexpr int kItems = 16;
-  for (int64_t start = 0; start < total; start += static_cast<int64_t>(kRsBlock) * kItems) {
-    int64_t base = start + static_cast<int64_t>(threadIdx.x) * kItems;
-    int32_t v[kItems];
-    int64_t s = 0;
-#pragma unroll
-    for (int i = 0; i < kItems; ++i) {
-      v[i] = (base + i < total) ? hist[base + i] : 0;
-      s += v[i];
-    }
-    int64_t tot;
-    int64_t ex = block_excl_scan(s, &tot) + carry_s;
-#pragma unroll
-    for (int i = 0; i < kItems; ++i) {
-      if (base + i < total) hist[base + i] = static_cast<int32_t>(ex);
-      ex += v[i];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) carry_s += tot;
-    __syncthreads();
-  }
-}
-
 template <typename K, typename V, bool HAS_V>
 __global__ __launch_bounds__(kRsBlock) void k_rs_scatter(const K* __restrict__ kin, const V* __restrict__ vin,
                                                          K* __restrict__ kout, V* __restrict__ vout, int64_t n,
                                                          int shift, int64_t ntiles,
-                                                         const int32_t* __restrict__ hist) {
+                                                         const int64_t* __restrict__ hoff) {
   __shared__ int32_t wcnt[kRsBlock / 64][kRsBins];
-  __shared__ int32_t goff[kRsBins];
+  __shared__ int64_t goff[kRsBins];
   const unsigned lane = __lane_id();
   const unsigned w = threadIdx.x >> 6;
   for (int i = threadIdx.x; i < (kRsBlock / 64) * kRsBins; i += kRsBlock) (&wcnt[0][0])[i] = 0;
-  for (int i = threadIdx.x; i < kRsBins; i += kRsBlock) goff[i] = hist[static_cast<int64_t>(i) * ntiles + blockIdx.x];
+  for (int i = threadIdx.x; i < kRsBins; i += kRsBlock) goff[i] = hoff[static_cast<int64_t>(i) * ntiles + blockIdx.x];
   __syncthreads();
   const int64_t wbase = static_cast<int64_t>(blockIdx.x) * kRsTile + static_cast<int64_t>(w) * kRsWaveKeys;
   K key[kRsChunks];
@@ -257,7 +228,7 @@ __global__ __launch_bounds__(kRsBlock) void k_rs_scatter(const K* __restrict__ k
   for (int c = 0; c < kRsChunks; ++c) {
     const int64_t idx = wbase + c * 64 + lane;
     if (idx < n) {
-      const int64_t pos = static_cast<int64_t>(goff[dig[c]]) + wcnt[w][dig[c]] + rank[c];
+      const int64_t pos = goff[dig[c]] + wcnt[w][dig[c]] + rank[c];
       kout[pos] = key[c];
       if (HAS_V) vout[pos] = val[c];
     }
@@ -266,14 +237,16 @@ __global__ __launch_bounds__(kRsBlock) void k_rs_scatter(const K* __restrict__ k
 
 template <typename K, typename V, bool HAS_V>
 static int radix_sort_t(hdb_ctx* ctx, K* keys, V* vals, int64_t n, int bit0, int bit1) {
-  if (n <= 1) return HDB_OK;
+  if (n <= 1 || bit1 <= bit0) return HDB_OK;
   const int64_t ntiles = (n + kRsTile - 1) / kRsTile;
   K* k2;
   V* v2 = nullptr;
   int32_t* hist;
+  int64_t* hoff;
   HDB_CALL(scratch_t(ctx, kSlotSortKeys, static_cast<size_t>(n), &k2));
   if (HAS_V) HDB_CALL(scratch_t(ctx, kSlotSortVals, static_cast<size_t>(n), &v2));
   HDB_CALL(scratch_t(ctx, kSlotSortHist, static_cast<size_t>(kRsBins * ntiles), &hist));
+  HDB_CALL(scratch_t(ctx, kSlotSortAux, static_cast<size_t>(kRsBins * ntiles + 1), &hoff));
   K* ka = keys;
   K* kb = k2;
   V* va = vals;
@@ -282,10 +255,9 @@ static int radix_sort_t(hdb_ctx* ctx, K* keys, V* vals, int64_t n, int bit0, int
   for (int shift = bit0; shift < bit1; shift += 8, ++passes) {
     hipLaunchKernelGGL(k_rs_hist<K>, dim3(ntiles), dim3(kRsBlock), 0, ctx->stream, ka, n, shift, ntiles, hist);
     HDB_LAUNCHED();
-    hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(kRsBlock), 0, ctx->stream, hist, kRsBins * ntiles);
-    HDB_LAUNCHED();
+    HDB_CALL(scan_i32_to_i64(ctx, hist, kRsBins * ntiles, hoff));
     hipLaunchKernelGGL((k_rs_scatter<K, V, HAS_V>), dim3(ntiles), dim3(kRsBlock), 0, ctx->stream, ka, va, kb, vb, n,
-                       shift, ntiles, hist);
+                       shift, ntiles, hoff);
     HDB_LAUNCHED();
     K* tk = ka; ka = kb; kb = tk;
     V* tv = va; va = vb; vb = tv;
@@ -300,8 +272,41 @@ static int radix_sort_t(hdb_ctx* ctx, K* keys, V* vals, int64_t n, int bit0, int
 int radix_sort_u32_u64(hdb_ctx* ctx, uint32_t* keys, uint64_t* vals, int64_t n, int bit0, int bit1) {
   return radix_sort_t<uint32_t, uint64_t, true>(ctx, keys, vals, n, bit0, bit1);
 }
+int radix_sort_u32_u32(hdb_ctx* ctx, uint32_t* keys, uint32_t* vals, int64_t n, int bit0, int bit1) {
+  return radix_sort_t<uint32_t, uint32_t, true>(ctx, keys, vals, n, bit0, bit1);
+}
 int radix_sort_u64(hdb_ctx* ctx, uint64_t* keys, int64_t n, int bit0, int bit1) {
   return radix_sort_t<uint64_t, uint64_t, false>(ctx, keys, static_cast<uint64_t*>(nullptr), n, bit0, bit1);
+}
+
+}  // namespace hdb
+
+namespace hdb {
+
+// -------------------------------------------------------- segment bounds --
+__global__ void k_seg_bounds(const uint32_t* __restrict__ keys, int64_t m, int64_t nseg, int32_t* __restrict__ s,
+                             int32_t* __restrict__ e) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const uint32_t k = keys[i];
+  if (static_cast<int64_t>(k) >= nseg) return;
+  if (i == 0 || keys[i - 1] != k) s[k] = static_cast<int32_t>(i);
+  if (i == m - 1 || keys[i + 1] != k) e[k] = static_cast<int32_t>(i + 1);
+}
+
+int segment_bounds(hdb_ctx* ctx, const uint32_t* keys, int64_t m, int64_t nseg, int32_t* s, int32_t* e) {
+  HDB_CALL(fill_i32(ctx, s, nseg, 0));
+  HDB_CALL(fill_i32(ctx, e, nseg, 0));
+  if (m <= 0) return HDB_OK;
+  hipLaunchKernelGGL(k_seg_bounds, dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream, keys, m, nseg, s, e);
+  HDB_LAUNCHED();
+  return HDB_OK;
+}
+
+int bits_for(int64_t max_key) {
+  int b = 0;
+  while (b < 63 && (static_cast<int64_t>(1) << b) <= max_key) ++b;
+  return b;
 }
 
 }  // namespace hdb

@@ -65,8 +65,13 @@ typedef struct {
   int32_t mpts;             /* core distance = entry mpts-1 of each list (32) */
   int32_t distance_mode;    /* 0: FP32 join; 1: BF16 MFMA join + FP32 re-rank */
   int32_t rerank_keep;      /* BF16 mode: survivors per candidate list (48) */
-  int32_t bucket_cap;       /* list-update proposals kept per list per pass (96) */
-  int32_t reserved[7];
+  int32_t bucket_cap;       /* proposal buffer capacity per iteration, in proposals
+                               per point (160); a workgroup whose proposals do not
+                               fit drops them and counts them */
+  int32_t join_keep;        /* local join: 0 = reference pairing (new: nearest new +
+                               nearest old partner; old: nearest new), T in 1..4 =
+                               every candidate proposes its T nearest partners (4) */
+  int32_t reserved[6];
 } hdb_params;
 
 typedef struct hdb_ctx hdb_ctx;
@@ -89,10 +94,12 @@ int hdb_timer_enable(hdb_ctx* ctx, int enable);
 int hdb_timer_reset(hdb_ctx* ctx);
 int hdb_timer_read(hdb_ctx* ctx, const char** names, double* ms, int64_t* launches,
                    int max_entries, int* n_entries);
-/* Counters of the last NN-Descent run: proposals made, proposals dropped at
- * full list buckets, reverse samples dropped at full reverse lists. */
+/* Counters of the last NN-Descent run, summed over its iterations:
+ * proposals made, proposals dropped because the proposal buffer was full,
+ * reverse pairs beyond the S kept per reverse list, and local-join
+ * candidates (sum over points of |new| + |old|). */
 int hdb_nnd_stats(hdb_ctx* ctx, int64_t* proposals, int64_t* bucket_drops,
-                  int64_t* reverse_drops);
+                  int64_t* reverse_drops, int64_t* candidates);
 
 /* ---- kNN graph (gpuknn/) ---------------------------------------------------- */
 /* Replaces gpuknn::NNDescent (gpuknn/nndescent.cuh:40-42, nndescent.cu:2088-2199):
