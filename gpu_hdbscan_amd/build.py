@@ -55,8 +55,14 @@ def build(verbose: bool = True) -> Path:
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", str(tmp), *objs])
     os.replace(tmp, LIB)
+    # C++ driver over include/gpu_hdbscan.hpp (mirrors HDBSCAN/_new.cu)
+    drv = PKG / "hdbscan_pipeline"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{PKG.parent / 'include'}", str(PKG / "cpp" / "hdbscan_pipeline.cpp"),
+          "-o", str(drv) + ".tmp", f"-L{PKG}", "-lhdbscan_mi355x", "-Wl,-rpath,$ORIGIN",
+          "-Wl,-rpath-link,/opt/rocm/lib"])
+    os.replace(str(drv) + ".tmp", drv)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {drv}")
     return LIB
 
 

@@ -1,0 +1,257 @@
+// gpu_hdbscan.hpp -- C++ host interface mirroring the reference's functions
+// for the hot path, implemented over the C ABI of libhdbscan_mi355x.so.
+//
+// Names, argument order and meaning follow the reference (gorlando04/
+// GPU_HDBSCAN): gpuknn::NNDescent (gpuknn/nndescent.cuh:40-42),
+// gpuknn::KNNMergeFromHost (gpuknn/knnmerge.cuh:13-17), calculate_degrees
+// (HDBSCAN/calculates/calculates.cuh:38), calculateCoreDistance (:26),
+// buildEnhancedKNNG (HDBSCAN/graphs/graph.cuh:24), gpuMST/buildMST
+// (HDBSCAN/mst/mst.cuh:64-91), sort_edges (HDBSCAN/merge_sort/merge_sort.cuh:30)
+// and get_clusters (HDBSCAN/trees/tree.cuh:33-47).  Differences:
+//   - errors throw hdb::Error (the reference prints and calls exit);
+//   - every function takes the hdb::Context to run on (the reference uses
+//     the current CUDA device and the default stream);
+//   - device memory comes from hdb_dev_alloc and is released with
+//     hdb::dev_free (the reference uses cudaMalloc/cudaMallocManaged);
+//   - buildEnhancedKNNG does not free its inputs (graph.cu:55,81 do).
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "hdbscan_mi355x.h"
+
+// Reference element layouts (tools/nndescent_element.cuh, HDBSCAN/structs).
+struct NNDElement {
+  float distance_;
+  int label_;
+  int label() const { return label_ >= 0 ? label_ : -label_ - 1; }
+  float distance() const { return distance_; }
+  bool IsNew() const { return label_ >= 0; }
+};
+static_assert(sizeof(NNDElement) == sizeof(hdb_nnd_element), "NNDElement layout");
+
+struct MSTedge {
+  int from_node;
+  int to_node;
+  float weight;
+};
+static_assert(sizeof(MSTedge) == sizeof(hdb_mst_edge), "MSTedge layout");
+
+struct ECLgraph {
+  int nodes = 0;
+  long edges = 0;
+  long* nindex = nullptr;  // host arrays (new[]); release with freeECLgraph
+  int* nlist = nullptr;
+  float* eweight = nullptr;
+};
+
+inline void freeECLgraph(ECLgraph& g) {
+  delete[] g.nindex;
+  delete[] g.nlist;
+  delete[] g.eweight;
+  g = ECLgraph();
+}
+
+namespace hdb {
+
+struct Error : std::runtime_error {
+  int status;
+  Error(int s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+class Context {
+ public:
+  explicit Context(int device = 0) {
+    int s = hdb_ctx_create(&ctx_, device);
+    if (s != HDB_OK) throw Error(s, "hdb_ctx_create failed");
+  }
+  ~Context() {
+    if (ctx_) hdb_ctx_destroy(ctx_);
+  }
+  Context(const Context&) = delete;
+  Context& operator=(const Context&) = delete;
+  hdb_ctx* get() const { return ctx_; }
+  void check(int s, int allowed = -1) const {
+    if (s != HDB_OK && s != allowed) throw Error(s, hdb_last_error(ctx_));
+  }
+  void* alloc(size_t bytes) const {
+    void* p = nullptr;
+    check(hdb_dev_alloc(ctx_, &p, bytes));
+    return p;
+  }
+  void free(void* p) const { check(hdb_dev_free(ctx_, p)); }
+  void h2d(void* dst, const void* src, size_t b) const { check(hdb_memcpy_h2d(ctx_, dst, src, b)); }
+  void d2h(void* dst, const void* src, size_t b) const { check(hdb_memcpy_d2h(ctx_, dst, src, b)); }
+
+ private:
+  hdb_ctx* ctx_ = nullptr;
+};
+
+inline hdb_params reference_params() {
+  hdb_params p;
+  hdb_params_default(&p);
+  return p;
+}
+
+template <typename T>
+struct DevBuf {
+  const Context& c;
+  T* p;
+  DevBuf(const Context& ctx, size_t n) : c(ctx), p(static_cast<T*>(ctx.alloc(n * sizeof(T)))) {}
+  ~DevBuf() { (void)hdb_dev_free(c.get(), p); }
+};
+
+}  // namespace hdb
+
+namespace gpuknn {
+
+// gpuknn::NNDescent (nndescent.cu:2088-2199): *knngraph_result_ptr receives
+// a device array (store_result_in_device) or a new[] host array of
+// vecs_size x 32 lists.
+inline void NNDescent(const hdb::Context& ctx, NNDElement** knngraph_result_ptr, const float* vectors_dev,
+                      int vecs_size, int vecs_dim, int iteration = 6, bool store_result_in_device = true) {
+  hdb_params p = hdb::reference_params();
+  p.nnd_iteration = iteration;
+  const size_t bytes = static_cast<size_t>(vecs_size) * 32 * sizeof(NNDElement);
+  auto* dev = static_cast<hdb_nnd_element*>(ctx.alloc(bytes));
+  ctx.check(hdb_nndescent(ctx.get(), vectors_dev, vecs_size, vecs_dim, &p, dev));
+  if (store_result_in_device) {
+    *knngraph_result_ptr = reinterpret_cast<NNDElement*>(dev);
+    return;
+  }
+  NNDElement* host = new NNDElement[static_cast<size_t>(vecs_size) * 32];
+  ctx.d2h(host, dev, bytes);
+  ctx.free(dev);
+  *knngraph_result_ptr = host;
+}
+
+// gpuknn::KNNMergeFromHost (knnmerge.cu:478-530): host vectors and graphs in,
+// merged graph (ids in [0, n1+n2)) out as a device array.
+inline void KNNMergeFromHost(const hdb::Context& ctx, NNDElement** knngraph_merged_dev_ptr,
+                             const float* vectors_first, int vectors_first_size, const NNDElement* knngraph_first,
+                             const float* vectors_second, int vectors_second_size,
+                             const NNDElement* knngraph_second, int dim) {
+  const size_t n = static_cast<size_t>(vectors_first_size) + vectors_second_size;
+  hdb::DevBuf<float> X(ctx, n * dim);
+  ctx.h2d(X.p, vectors_first, static_cast<size_t>(vectors_first_size) * dim * sizeof(float));
+  ctx.h2d(X.p + static_cast<size_t>(vectors_first_size) * dim, vectors_second,
+          static_cast<size_t>(vectors_second_size) * dim * sizeof(float));
+  hdb::DevBuf<hdb_nnd_element> g1(ctx, static_cast<size_t>(vectors_first_size) * 32);
+  hdb::DevBuf<hdb_nnd_element> g2(ctx, static_cast<size_t>(vectors_second_size) * 32);
+  ctx.h2d(g1.p, knngraph_first, static_cast<size_t>(vectors_first_size) * 32 * sizeof(NNDElement));
+  ctx.h2d(g2.p, knngraph_second, static_cast<size_t>(vectors_second_size) * 32 * sizeof(NNDElement));
+  auto* out = static_cast<hdb_nnd_element*>(ctx.alloc(n * 32 * sizeof(NNDElement)));
+  hdb_params p = hdb::reference_params();
+  ctx.check(hdb_knn_merge(ctx.get(), X.p, vectors_first_size, vectors_second_size, dim, g1.p, g2.p, &p, out));
+  *knngraph_merged_dev_ptr = reinterpret_cast<NNDElement*>(out);
+}
+
+}  // namespace gpuknn
+
+// GenLargeKNNGraph (gen_large_knngraph.cu:409-484) for one device and data
+// already in HBM: blocked NN-Descent + all-pairs knnmerge.
+inline void GenLargeKNNGraph(const hdb::Context& ctx, const float* vectors_dev, int n, int dim, int num_shards,
+                             NNDElement* out_dev) {
+  hdb_params p = hdb::reference_params();
+  ctx.check(hdb_build_knng_blocked(ctx.get(), vectors_dev, n, dim, num_shards, &p,
+                                   reinterpret_cast<hdb_nnd_element*>(out_dev)));
+}
+
+// calculate_degrees (calculates.cu:448-473): in-degree of every vertex over
+// the device kNN lists (the reference takes the label array kNN[n*k]).
+inline std::vector<int> calculate_degrees(const hdb::Context& ctx, const NNDElement* knn_dev, long numValues) {
+  hdb::DevBuf<int32_t> c(ctx, numValues);
+  ctx.check(hdb_hub_counts(ctx.get(), reinterpret_cast<const hdb_nnd_element*>(knn_dev), numValues, 32, c.p));
+  std::vector<int> out(numValues);
+  ctx.d2h(out.data(), c.p, numValues * sizeof(int));
+  return out;
+}
+
+// calculateCoreDistance (calculates.cu:185-197) at list entry mpts-1.
+inline std::vector<float> calculateCoreDistance(const hdb::Context& ctx, const float* vectors_dev,
+                                                const NNDElement* knn_dev, long numValues, int dim, long mpts) {
+  hdb::DevBuf<float> c(ctx, numValues);
+  ctx.check(hdb_core_distances(ctx.get(), vectors_dev, reinterpret_cast<const hdb_nnd_element*>(knn_dev), numValues,
+                               dim, 32, static_cast<int>(mpts), c.p));
+  std::vector<float> out(numValues);
+  ctx.d2h(out.data(), c.p, numValues * sizeof(float));
+  return out;
+}
+
+// buildEnhancedKNNG (graph.cu:108-176): hub counts -> anti-hubs -> CSR ->
+// core distances -> MRD weights (+ anti-hub clique), as a host ECLgraph.
+inline ECLgraph buildEnhancedKNNG(const hdb::Context& ctx, const float* vectors_dev, const NNDElement* knn_dev,
+                                  long numValues, int dim, long mpts) {
+  auto* L = reinterpret_cast<const hdb_nnd_element*>(knn_dev);
+  hdb::DevBuf<int32_t> counts(ctx, numValues);
+  ctx.check(hdb_hub_counts(ctx.get(), L, numValues, 32, counts.p));
+  const long hcap = static_cast<long>(std::sqrt(static_cast<double>(numValues))) + 2;
+  hdb::DevBuf<int32_t> ah(ctx, hcap);
+  int64_t h = 0;
+  ctx.check(hdb_antihubs(ctx.get(), L, counts.p, numValues, 32, ah.p, &h));
+  hdb::DevBuf<float> core(ctx, numValues);
+  ctx.check(hdb_core_distances(ctx.get(), vectors_dev, L, numValues, dim, 32, static_cast<int>(mpts), core.p));
+  hdb::DevBuf<int64_t> nindex(ctx, numValues + 1);
+  int64_t E = 0;
+  ctx.check(hdb_mrd_graph(ctx.get(), vectors_dev, L, core.p, ah.p, h, numValues, dim, 32, nindex.p, nullptr, nullptr,
+                          &E));
+  hdb::DevBuf<int32_t> nlist(ctx, E > 0 ? E : 1);
+  hdb::DevBuf<float> w(ctx, E > 0 ? E : 1);
+  ctx.check(hdb_mrd_graph(ctx.get(), vectors_dev, L, core.p, ah.p, h, numValues, dim, 32, nindex.p, nlist.p, w.p, &E));
+  ECLgraph g;
+  g.nodes = static_cast<int>(numValues);
+  g.edges = static_cast<long>(E);
+  g.nindex = new long[numValues + 1];
+  g.nlist = new int[E > 0 ? E : 1];
+  g.eweight = new float[E > 0 ? E : 1];
+  ctx.d2h(g.nindex, nindex.p, (numValues + 1) * sizeof(long));
+  ctx.d2h(g.nlist, nlist.p, E * sizeof(int));
+  ctx.d2h(g.eweight, w.p, E * sizeof(float));
+  return g;
+}
+
+// gpuMST + buildMST (mst.cu:210-334): MST edges (CSR order), n-1 of them
+// when the graph is connected; *components receives the tree count (the
+// reference exits when the MST does not have n-1 edges).
+inline std::vector<MSTedge> buildMST(const hdb::Context& ctx, const ECLgraph& g, long* components = nullptr) {
+  hdb::DevBuf<int64_t> ni(ctx, g.nodes + 1);
+  hdb::DevBuf<int32_t> nl(ctx, g.edges > 0 ? g.edges : 1);
+  hdb::DevBuf<float> w(ctx, g.edges > 0 ? g.edges : 1);
+  static_assert(sizeof(long) == sizeof(int64_t), "LP64");
+  ctx.h2d(ni.p, g.nindex, (g.nodes + 1) * sizeof(int64_t));
+  ctx.h2d(nl.p, g.nlist, g.edges * sizeof(int));
+  ctx.h2d(w.p, g.eweight, g.edges * sizeof(float));
+  hdb::DevBuf<hdb_mst_edge> out(ctx, g.nodes > 1 ? g.nodes - 1 : 1);
+  int64_t m = 0, comps = 0;
+  ctx.check(hdb_mst(ctx.get(), ni.p, nl.p, w.p, g.nodes, out.p, &m, &comps), HDB_EDISCONNECTED);
+  std::vector<MSTedge> e(m);
+  ctx.d2h(e.data(), out.p, m * sizeof(MSTedge));
+  if (components) *components = static_cast<long>(comps);
+  return e;
+}
+
+// sort_edges (merge_sort.cu:226-279): stable ascending sort by weight.
+inline void sort_edges(const hdb::Context& ctx, std::vector<MSTedge>& edges) {
+  if (edges.size() < 2) return;
+  hdb::DevBuf<hdb_mst_edge> d(ctx, edges.size());
+  ctx.h2d(d.p, edges.data(), edges.size() * sizeof(MSTedge));
+  ctx.check(hdb_sort_edges(ctx.get(), d.p, static_cast<int64_t>(edges.size())));
+  ctx.d2h(edges.data(), d.p, edges.size() * sizeof(MSTedge));
+}
+
+// build_Linkage_tree .. get_clusters (tree.cu:171-746): labels of numValues
+// points from the n-1 sorted MST edges (noise = -1).
+inline std::vector<int> get_clusters(const std::vector<MSTedge>& sorted_edges, long numValues, int mpts,
+                                     int* n_clusters = nullptr) {
+  std::vector<int> labels(numValues);
+  int nc = 0;
+  int s = hdb_labels(reinterpret_cast<const hdb_mst_edge*>(sorted_edges.data()), numValues, mpts, labels.data(), &nc);
+  if (s != HDB_OK) throw hdb::Error(s, "hdb_labels failed");
+  if (n_clusters) *n_clusters = nc;
+  return labels;
+}
