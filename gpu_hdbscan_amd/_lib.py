@@ -84,6 +84,15 @@ PROTOTYPES = {
     "hdb_pipeline": (C.c_int, [VP, VP, I64, I32, C.POINTER(HdbParams), C.POINTER(HdbPipelineOpts), VP, VP, VP,
                                VP, PI64, PI64, PI64]),
     "hdb_labels": (C.c_int, [VP, I64, I32, VP, PI32]),
+    "hdb_mlp_create": (C.c_int, [VP, I32, C.c_uint64, C.POINTER(VP)]),
+    "hdb_mlp_destroy": (C.c_int, [VP]),
+    "hdb_mlp_param_count": (C.c_int64, [VP]),
+    "hdb_mlp_params": (C.c_int, [VP, VP, VP, VP]),
+    "hdb_mlp_forward": (C.c_int, [VP, VP, VP, I64, I32, VP]),
+    "hdb_mlp_loss_grad": (C.c_int, [VP, VP, VP, I64, I32, VP, I64, C.POINTER(C.c_double), VP]),
+    "hdb_mlp_train": (C.c_int, [VP, VP, VP, I64, I32, VP, I32, I32, C.c_float, C.c_uint64,
+                                C.POINTER(C.c_double)]),
+    "hdb_learned_init": (C.c_int, [VP, VP, VP, I64, I32, C.c_uint64, VP]),
 }
 
 _lib = None

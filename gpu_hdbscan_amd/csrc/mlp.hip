@@ -1,0 +1,638 @@
+// Learned initialiser for NN-Descent on gfx950 (no reference counterpart: it
+// replaces the random index init of InitKNNGraphIndexKernel,
+// gpuknn/nndescent.cu:1673-1692, with bucket co-members).
+//
+// Network: d_in -> 128 -> 128 -> 12, GELU after the two hidden layers.  Every
+// product runs on v_mfma_f32_16x16x32_bf16: operands (inputs, activations,
+// gradients and the FP32 master weights) are rounded to BF16 when the
+// fragments are formed, accumulation is FP32.  Lane layout of the
+// instruction: A/B lane l holds row (l % 16), k = 8*(l/16) .. +7; the result
+// lane l holds rows 4*(l/16) + q (q = 0..3), column l % 16.
+//
+// Training pairs: anchor a, positive = a kNN neighbour of a (from a plain
+// NN-Descent graph of a subsample), negative = a random point.  With
+// t = tanh(z):  L = mean_batch[ |t_a - t_p|^2 / 12 + (t_a . t_n / 12)^2 ]
+//                   + sum_bits mean_rows(t)^2
+// (agreeing signs for neighbours, uncorrelated codes for random pairs,
+// balanced bits).  The 12 sign bits of z are the bucket code.
+#include <algorithm>
+#include <cmath>
+
+#include "hdb_internal.hpp"
+
+namespace hdb {
+
+constexpr int kHid = 128;
+constexpr int kCode = 12;
+constexpr int kOutP = 16;  // padded output width
+constexpr int kMlpWaves = 2;  // waves per forward block (LDS: 16 rows x (d_in + 2 x 128) floats each)
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+struct MlpLayout {
+  int kin, kinp;  // input width and its multiple of 32
+  int64_t w1, b1, w2, b2, w3, b3, total;
+};
+
+static MlpLayout mlp_layout(int kin) {
+  MlpLayout L;
+  L.kin = kin;
+  L.kinp = (kin + 31) & ~31;
+  L.w1 = 0;
+  L.b1 = L.w1 + static_cast<int64_t>(L.kinp) * kHid;
+  L.w2 = L.b1 + kHid;
+  L.b2 = L.w2 + kHid * kHid;
+  L.w3 = L.b2 + kHid;
+  L.b3 = L.w3 + kHid * kOutP;
+  L.total = L.b3 + kOutP;
+  return L;
+}
+
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float x) {
+  const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
+  const float pdf = 0.39894228040143268f * __expf(-0.5f * x * x);
+  return cdf + x * pdf;
+}
+
+__device__ __forceinline__ f32x4 mfma(const bf16x8& a, const bf16x8& b, const f32x4& c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// One wave computes out[16 x N] = act(in[16 x K] . W[K x N] + bias) with
+// `in` a row-major LDS tile (stride ldi) and W row-major in global memory
+// (N columns, stride N).  Results go to LDS (stride ldo) and optionally to
+// global pre-activation / activation rows.
+template <bool GELU>
+__device__ __forceinline__ void wave_layer(const float* in, int ldi, int K, const float* __restrict__ W,
+                                           const float* __restrict__ bias, int N, float* out, int ldo,
+                                           float* gpre, float* gact, int64_t grow0, int64_t rows_valid) {
+  const unsigned l = __lane_id();
+  const int r16 = l & 15, kq = l >> 4;
+  for (int n0 = 0; n0 < N; n0 += 16) {
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k0 = 0; k0 < K; k0 += 32) {
+      bf16x8 a, b;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int k = k0 + 8 * kq + e;
+        a[e] = static_cast<__bf16>(in[r16 * ldi + k]);
+        b[e] = static_cast<__bf16>(W[static_cast<int64_t>(k) * N + n0 + r16]);
+      }
+      acc = mfma(a, b, acc);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int row = 4 * kq + q, col = n0 + r16;
+      const float pre = acc[q] + bias[col];
+      const float v = GELU ? gelu(pre) : pre;
+      out[row * ldo + col] = v;
+      if (row < rows_valid) {
+        if (gpre) gpre[(grow0 + row) * N + col] = pre;
+        if (gact) gact[(grow0 + row) * N + col] = v;
+      }
+    }
+  }
+}
+
+// Forward pass over R rows (row r = point rows[r], or r when rows is null).
+// TRAIN stores P1, H1, P2, H2 (R x 128) for the backward pass.  Z: R x 16.
+template <bool TRAIN>
+__global__ __launch_bounds__(kMlpWaves * 64) void k_mlp_fwd(const float* __restrict__ X, int d, const int32_t* __restrict__ rows,
+                                                 int64_t R, const float* __restrict__ prm, MlpLayout L,
+                                                 float* __restrict__ P1, float* __restrict__ H1,
+                                                 float* __restrict__ P2, float* __restrict__ H2,
+                                                 float* __restrict__ Z) {
+  extern __shared__ float lds[];
+  const unsigned l = __lane_id();
+  const unsigned w = threadIdx.x >> 6;
+  const int ldi = L.kinp + 4, lda = kHid + 4;
+  float* in = lds + w * (16 * ldi + 2 * 16 * lda);
+  float* a1 = in + 16 * ldi;
+  float* a2 = a1 + 16 * lda;
+  float* zo = in;  // the input tile is dead once layer 1 is done
+  const int64_t r0 = (static_cast<int64_t>(blockIdx.x) * kMlpWaves + w) * 16;
+  if (r0 >= R) return;  // no block barriers in this kernel
+  const int64_t valid = R - r0 < 16 ? R - r0 : 16;
+  for (int i = 0; i < 16; ++i) {
+    const bool ok = i < valid;
+    const int64_t src = ok ? (rows ? rows[r0 + i] : r0 + i) : 0;
+    for (int k = l; k < L.kinp; k += 64) in[i * ldi + k] = (ok && k < d) ? X[src * d + k] : 0.0f;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  wave_layer<true>(in, ldi, L.kinp, prm + L.w1, prm + L.b1, kHid, a1, lda, TRAIN ? P1 : nullptr,
+                   TRAIN ? H1 : nullptr, r0, valid);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  wave_layer<true>(a1, lda, kHid, prm + L.w2, prm + L.b2, kHid, a2, lda, TRAIN ? P2 : nullptr,
+                   TRAIN ? H2 : nullptr, r0, valid);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  wave_layer<false>(a2, lda, kHid, prm + L.w3, prm + L.b3, kOutP, zo, kOutP + 4, nullptr, Z, r0, valid);
+}
+
+// Backward over rows: dH2 = dZ W3^T, dP2 = dH2 * gelu'(P2), dH1 = dP2 W2^T,
+// dP1 = dH1 * gelu'(P1).  Per wave 16 rows.
+__global__ __launch_bounds__(256) void k_mlp_bwd_rows(int64_t R, const float* __restrict__ prm, MlpLayout L,
+                                                      const float* __restrict__ P1, const float* __restrict__ P2,
+                                                      const float* __restrict__ dZ, float* __restrict__ dP2,
+                                                      float* __restrict__ dP1) {
+  extern __shared__ float lds[];
+  const unsigned l = __lane_id();
+  const unsigned w = threadIdx.x >> 6;
+  const int lda = kHid + 4;
+  float* gz = lds + w * (16 * 36 + 16 * lda);  // dZ tile padded to K = 32
+  float* g2 = gz + 16 * 36;
+  const int64_t r0 = (static_cast<int64_t>(blockIdx.x) * 4 + w) * 16;
+  if (r0 >= R) return;
+  const int64_t valid = R - r0 < 16 ? R - r0 : 16;
+  const int r16 = l & 15, kq = l >> 4;
+  for (int i = l; i < 16 * 32; i += 64) {
+    const int row = i >> 5, c = i & 31;
+    gz[row * 36 + c] = (row < valid && c < kOutP) ? dZ[(r0 + row) * kOutP + c] : 0.0f;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  const float* W3 = prm + L.w3;  // [128][16]
+  const float* W2 = prm + L.w2;  // [128][128]
+  for (int n0 = 0; n0 < kHid; n0 += 16) {
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    bf16x8 a, b;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int k = 8 * kq + e;  // output unit of layer 3
+      a[e] = static_cast<__bf16>(gz[r16 * 36 + k]);
+      b[e] = static_cast<__bf16>(k < kOutP ? W3[(n0 + r16) * kOutP + k] : 0.0f);
+    }
+    acc = mfma(a, b, acc);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int row = 4 * kq + q, col = n0 + r16;
+      const float pre = row < valid ? P2[(r0 + row) * kHid + col] : 0.0f;
+      const float g = acc[q] * gelu_grad(pre);
+      g2[row * lda + col] = row < valid ? g : 0.0f;
+      if (row < valid) dP2[(r0 + row) * kHid + col] = g;
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  for (int n0 = 0; n0 < kHid; n0 += 16) {
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k0 = 0; k0 < kHid; k0 += 32) {
+      bf16x8 a, b;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int k = k0 + 8 * kq + e;  // unit of hidden layer 2
+        a[e] = static_cast<__bf16>(g2[r16 * lda + k]);
+        b[e] = static_cast<__bf16>(W2[(n0 + r16) * kHid + k]);
+      }
+      acc = mfma(a, b, acc);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int row = 4 * kq + q, col = n0 + r16;
+      if (row < valid) dP1[(r0 + row) * kHid + col] = acc[q] * gelu_grad(P1[(r0 + row) * kHid + col]);
+    }
+  }
+}
+
+// Weight gradients G[m][n] = sum_r A[r][m] * B[r][n] over R rows, split in
+// chunks of `chunk` rows (gridDim.z chunks); partials[z][M][N].  A rows come
+// from X (gathered, width d, zero past d) when rows != null, else from a
+// dense R x lda matrix.
+__global__ __launch_bounds__(64) void k_mlp_wgrad(const float* __restrict__ A, int lda, const float* __restrict__ X,
+                                                  int d, const int32_t* __restrict__ rows, const float* __restrict__ B,
+                                                  int ldb, int64_t R, int M, int N, int64_t chunk,
+                                                  float* __restrict__ part) {
+  const unsigned l = __lane_id();
+  const int r16 = l & 15, kq = l >> 4;
+  const int n0 = blockIdx.x * 16, m0 = blockIdx.y * 16;
+  const int64_t rbeg = blockIdx.z * chunk;
+  const int64_t rend = rbeg + chunk < R ? rbeg + chunk : R;
+  f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int64_t k0 = rbeg; k0 < rend; k0 += 32) {
+    bf16x8 a, b;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int64_t r = k0 + 8 * kq + e;
+      float av = 0.0f, bv = 0.0f;
+      if (r < rend) {
+        const int m = m0 + r16;
+        if (rows) av = m < d ? X[static_cast<int64_t>(rows[r]) * d + m] : 0.0f;
+        else av = A[r * lda + m];
+        bv = B[r * ldb + n0 + r16];
+      }
+      a[e] = static_cast<__bf16>(av);
+      b[e] = static_cast<__bf16>(bv);
+    }
+    acc = mfma(a, b, acc);
+  }
+  float* out = part + static_cast<int64_t>(blockIdx.z) * M * N;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) out[(m0 + 4 * kq + q) * N + n0 + r16] = acc[q];
+}
+
+// grad[i] = sum_z part[z][i] (fixed order), i < M*N.
+__global__ void k_mlp_reduce(const float* __restrict__ part, int64_t splits, int64_t mn, float* __restrict__ grad) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= mn) return;
+  float s = 0.0f;
+  for (int64_t z = 0; z < splits; ++z) s += part[z * mn + i];
+  grad[i] = s;
+}
+
+// Bias gradients: column sums of a dense R x n matrix (one thread per column).
+__global__ void k_mlp_colsum(const float* __restrict__ G, int64_t R, int n, float* __restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  float s = 0.0f;
+  for (int64_t r = 0; r < R; ++r) s += G[r * n + c];
+  out[c] = s;
+}
+
+// Mean of tanh(z) per code bit over the R rows (one block).
+__global__ __launch_bounds__(256) void k_mlp_tmean(const float* __restrict__ Z, int64_t R, float* __restrict__ mean) {
+  __shared__ float acc[kCode][256];
+  float s[kCode];
+  for (int b = 0; b < kCode; ++b) s[b] = 0.0f;
+  for (int64_t r = threadIdx.x; r < R; r += 256)
+    for (int b = 0; b < kCode; ++b) s[b] += tanhf(Z[r * kOutP + b]);
+  for (int b = 0; b < kCode; ++b) acc[b][threadIdx.x] = s[b];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (static_cast<int>(threadIdx.x) < st)
+      for (int b = 0; b < kCode; ++b) acc[b][threadIdx.x] += acc[b][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x < kCode) mean[threadIdx.x] = acc[threadIdx.x][0] / static_cast<float>(R);
+}
+
+// Loss and dZ for B (anchor, positive, negative) triples stored as rows
+// [0,B), [B,2B), [2B,3B).  loss_part[b] = per-triple loss (balance added
+// once by the host from `mean`).
+__global__ void k_mlp_loss(const float* __restrict__ Z, int64_t B, const float* __restrict__ mean,
+                           float* __restrict__ dZ, float* __restrict__ loss_part) {
+  const int64_t b = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const float* za = Z + b * kOutP;
+  const float* zp = Z + (B + b) * kOutP;
+  const float* zn = Z + (2 * B + b) * kOutP;
+  float ta[kCode], tp[kCode], tn[kCode];
+  float lpos = 0.0f, s = 0.0f;
+  for (int i = 0; i < kCode; ++i) {
+    ta[i] = tanhf(za[i]);
+    tp[i] = tanhf(zp[i]);
+    tn[i] = tanhf(zn[i]);
+    const float df = ta[i] - tp[i];
+    lpos += df * df;
+    s += ta[i] * tn[i];
+  }
+  lpos /= kCode;
+  s /= kCode;
+  loss_part[b] = lpos + s * s;
+  const float inv_b = 1.0f / static_cast<float>(B);
+  const float bal = 2.0f / static_cast<float>(3 * B);
+  float* ga = dZ + b * kOutP;
+  float* gp = dZ + (B + b) * kOutP;
+  float* gn = dZ + (2 * B + b) * kOutP;
+  for (int i = 0; i < kCode; ++i) {
+    const float dta = (2.0f / kCode) * (ta[i] - tp[i]) * inv_b + (2.0f * s / kCode) * tn[i] * inv_b + bal * mean[i];
+    const float dtp = -(2.0f / kCode) * (ta[i] - tp[i]) * inv_b + bal * mean[i];
+    const float dtn = (2.0f * s / kCode) * ta[i] * inv_b + bal * mean[i];
+    ga[i] = dta * (1.0f - ta[i] * ta[i]);
+    gp[i] = dtp * (1.0f - tp[i] * tp[i]);
+    gn[i] = dtn * (1.0f - tn[i] * tn[i]);
+  }
+  for (int i = kCode; i < kOutP; ++i) ga[i] = gp[i] = gn[i] = 0.0f;
+}
+
+// Triples for step `step`: anchor uniform, positive = a uniform entry of the
+// anchor's kNN list, negative uniform (xorshift64* streams).
+__global__ void k_mlp_batch(const hdb_nnd_element* __restrict__ lists, int64_t n, int64_t B, uint64_t seed,
+                            int32_t* __restrict__ rows) {
+  const int64_t b = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const uint64_t h0 = xorshift64star(seed ^ (static_cast<uint64_t>(b) * 0x9E3779B97F4A7C15ull));
+  const uint64_t h1 = xorshift64star(h0);
+  const uint64_t h2 = xorshift64star(h1);
+  const int64_t a = static_cast<int64_t>(h0 % static_cast<uint64_t>(n));
+  uint32_t pid = label_id(lists[a * kK + static_cast<int>(h1 % kK)].label);
+  if (pid == kInvalidId || static_cast<int64_t>(pid) >= n) pid = static_cast<uint32_t>(a);
+  rows[b] = static_cast<int32_t>(a);
+  rows[B + b] = static_cast<int32_t>(pid);
+  rows[2 * B + b] = static_cast<int32_t>(h2 % static_cast<uint64_t>(n));
+}
+
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, int64_t np, float lr, float c1, float c2, const uint8_t* __restrict__ mask) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= np || !mask[i]) return;
+  const float gi = g[i];
+  const float mi = 0.9f * m[i] + 0.1f * gi;
+  const float vi = 0.999f * v[i] + 0.001f * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
+}
+
+// Sort keys of the bucket order: (12-bit code << 20) | 20-bit rank of x.u
+// (u a fixed Gaussian direction, x.u clamped to [-lim, lim]).
+__global__ void k_mlp_keys(const float* __restrict__ Z, const float* __restrict__ X, int d,
+                           const float* __restrict__ u, float lim, int64_t n, uint32_t* __restrict__ keys,
+                           uint32_t* __restrict__ ids) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t code = 0;
+  for (int b = 0; b < kCode; ++b) code |= (Z[i * kOutP + b] > 0.0f ? 1u : 0u) << b;
+  float s = 0.0f;
+  for (int k = 0; k < d; ++k) s = __builtin_fmaf(X[i * d + k], u[k], s);
+  float t = (s / lim) * 0.5f + 0.5f;
+  t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+  const uint32_t q = static_cast<uint32_t>(t * 1048575.0f);
+  keys[i] = (code << 20) | q;
+  ids[i] = static_cast<uint32_t>(i);
+}
+
+// Initial lists from the bucket order: the point at sorted position r takes
+// the points at positions r-16 .. r+16 (shifted inside [0, n)), self excluded.
+__global__ void k_mlp_windows(const uint32_t* __restrict__ order, int64_t n, hdb_nnd_element* __restrict__ lists) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t r = t >> 5;
+  if (r >= n) return;
+  const int j = static_cast<int>(t & 31);
+  int64_t lo = r - 16;
+  if (lo < 0) lo = 0;
+  if (lo + 32 >= n) lo = n - 33 < 0 ? 0 : n - 33;
+  int64_t pos = lo + j;
+  if (pos >= r) pos += 1;
+  const uint32_t self = order[r];
+  hdb_nnd_element e;
+  e.distance = 0.0f;
+  e.label = pos < n ? static_cast<int32_t>(order[pos]) : static_cast<int32_t>(kInvalidId);
+  lists[static_cast<int64_t>(self) * kK + j] = e;
+}
+
+}  // namespace hdb
+
+// ------------------------------------------------------------- C ABI --
+using namespace hdb;
+
+struct hdb_mlp {
+  MlpLayout L;
+  float* prm = nullptr;   // device: parameters (FP32 master)
+  float* grad = nullptr;  // device: gradient
+  float* m = nullptr;
+  float* v = nullptr;
+  uint8_t* mask = nullptr;  // trainable entries (zero padding excluded)
+  int64_t step = 0;
+};
+
+namespace {
+
+int mlp_alloc(hdb_ctx* ctx, void** p, size_t bytes) {
+  if (hipMalloc(p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return ctx->fail(HDB_ENOMEM, "MLP allocation failed");
+  }
+  return HDB_OK;
+}
+
+// bytes of LDS for the forward kernel
+size_t fwd_lds(const MlpLayout& L) {
+  return static_cast<size_t>(kMlpWaves) * (16 * (L.kinp + 4) + 2 * 16 * (kHid + 4)) * sizeof(float);
+}
+
+int mlp_forward(hdb_ctx* ctx, hdb_mlp* net, const float* X, int d, const int32_t* rows, int64_t R, float* P1,
+                float* H1, float* P2, float* H2, float* Z) {
+  const unsigned grid = grid_for(R, 16 * kMlpWaves);
+  if (P1)
+    hipLaunchKernelGGL(k_mlp_fwd<true>, dim3(grid), dim3(kMlpWaves * 64), fwd_lds(net->L), ctx->stream, X, d, rows, R, net->prm,
+                       net->L, P1, H1, P2, H2, Z);
+  else
+    hipLaunchKernelGGL(k_mlp_fwd<false>, dim3(grid), dim3(kMlpWaves * 64), fwd_lds(net->L), ctx->stream, X, d, rows, R, net->prm,
+                       net->L, P1, H1, P2, H2, Z);
+  HDB_LAUNCHED();
+  return HDB_OK;
+}
+
+// Weight-gradient GEMM M x N over R rows into grad (M*N floats).
+int wgrad(hdb_ctx* ctx, const float* A, int lda, const float* X, int d, const int32_t* rows, const float* B, int ldb,
+          int64_t R, int M, int N, float* part, float* grad) {
+  const int64_t chunk = 256;
+  const int64_t splits = (R + chunk - 1) / chunk;
+  hipLaunchKernelGGL(k_mlp_wgrad, dim3(N / 16, M / 16, static_cast<unsigned>(splits)), dim3(64), 0, ctx->stream, A,
+                     lda, X, d, rows, B, ldb, R, M, N, chunk, part);
+  HDB_LAUNCHED();
+  hipLaunchKernelGGL(k_mlp_reduce, dim3(grid_for(static_cast<int64_t>(M) * N, 256)), dim3(256), 0, ctx->stream, part,
+                     splits, static_cast<int64_t>(M) * N, grad);
+  HDB_LAUNCHED();
+  return HDB_OK;
+}
+
+// One loss + gradient evaluation for the triples in `rows` (3B rows).
+int mlp_loss_grad(hdb_ctx* ctx, hdb_mlp* net, const float* X, int d, const int32_t* rows, int64_t B, double* loss) {
+  const MlpLayout& L = net->L;
+  const int64_t R = 3 * B;
+  float *P1, *H1, *P2, *H2, *Z, *dZ, *dP2, *dP1, *part, *mean, *lossp;
+  HDB_CALL(scratch_t(ctx, kSlotMisc0, 4 * R * kHid, &P1));
+  H1 = P1 + R * kHid;
+  P2 = H1 + R * kHid;
+  H2 = P2 + R * kHid;
+  HDB_CALL(scratch_t(ctx, kSlotMisc1, 2 * R * kHid, &dP2));
+  dP1 = dP2 + R * kHid;
+  HDB_CALL(scratch_t(ctx, kSlotMisc2, 2 * R * kOutP + B + 64, &Z));
+  dZ = Z + R * kOutP;
+  lossp = dZ + R * kOutP;
+  mean = lossp + B;
+  const int64_t splits = (R + 255) / 256;
+  HDB_CALL(scratch_t(ctx, kSlotBfTmp, splits * std::max(L.kinp, kHid) * kHid, &part));
+  HDB_CALL(mlp_forward(ctx, net, X, d, rows, R, P1, H1, P2, H2, Z));
+  hipLaunchKernelGGL(k_mlp_tmean, dim3(1), dim3(256), 0, ctx->stream, Z, R, mean);
+  HDB_LAUNCHED();
+  hipLaunchKernelGGL(k_mlp_loss, dim3(grid_for(B, 256)), dim3(256), 0, ctx->stream, Z, B, mean, dZ, lossp);
+  HDB_LAUNCHED();
+  const size_t blds = static_cast<size_t>(4) * (16 * 36 + 16 * (kHid + 4)) * sizeof(float);
+  hipLaunchKernelGGL(k_mlp_bwd_rows, dim3(grid_for(R, 64)), dim3(256), blds, ctx->stream, R, net->prm, L, P1, P2, dZ,
+                     dP2, dP1);
+  HDB_LAUNCHED();
+  float* g = net->grad;
+  HDB_CALL(wgrad(ctx, H2, kHid, nullptr, 0, nullptr, dZ, kOutP, R, kHid, kOutP, part, g + L.w3));
+  HDB_CALL(wgrad(ctx, H1, kHid, nullptr, 0, nullptr, dP2, kHid, R, kHid, kHid, part, g + L.w2));
+  HDB_CALL(wgrad(ctx, nullptr, 0, X, d, rows, dP1, kHid, R, L.kinp, kHid, part, g + L.w1));
+  hipLaunchKernelGGL(k_mlp_colsum, dim3(1), dim3(kOutP), 0, ctx->stream, dZ, R, kOutP, g + L.b3);
+  HDB_LAUNCHED();
+  hipLaunchKernelGGL(k_mlp_colsum, dim3(1), dim3(kHid), 0, ctx->stream, dP2, R, kHid, g + L.b2);
+  HDB_LAUNCHED();
+  hipLaunchKernelGGL(k_mlp_colsum, dim3(1), dim3(kHid), 0, ctx->stream, dP1, R, kHid, g + L.b1);
+  HDB_LAUNCHED();
+  if (loss) {
+    std::vector<float> lp(B);
+    float mh[kCode];
+    HDB_HIP(hipMemcpyAsync(lp.data(), lossp, B * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HDB_HIP(hipMemcpyAsync(mh, mean, sizeof(mh), hipMemcpyDeviceToHost, ctx->stream));
+    HDB_HIP(hipStreamSynchronize(ctx->stream));
+    double s = 0.0;
+    for (float x : lp) s += x;
+    s /= static_cast<double>(B);
+    for (float x : mh) s += static_cast<double>(x) * x;
+    *loss = s;
+  }
+  return HDB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hdb_mlp_create(hdb_ctx* ctx, int32_t d_in, uint64_t seed, hdb_mlp** out) {
+  if (!ctx || !out || d_in < 1 || d_in > 128) return ctx ? ctx->fail(HDB_EINVAL, "MLP input width must be 1..128") : HDB_EINVAL;
+  HDB_HIP(hipSetDevice(ctx->device));
+  hdb_mlp* net = new hdb_mlp();
+  net->L = mlp_layout(d_in);
+  const MlpLayout& L = net->L;
+  std::vector<float> h(L.total, 0.0f);
+  std::vector<uint8_t> mask(L.total, 0);
+  // He-style uniform init from xorshift64*, zero biases; padding stays zero
+  uint64_t s = seed * 0x9E3779B97F4A7C15ull + 1;
+  auto uni = [&s]() {
+    s = xorshift64star(s);
+    return static_cast<float>((s >> 11) * (1.0 / 9007199254740992.0)) * 2.0f - 1.0f;
+  };
+  const float a1 = std::sqrt(6.0f / d_in), a2 = std::sqrt(6.0f / kHid);
+  for (int k = 0; k < L.kinp; ++k)
+    for (int j = 0; j < kHid; ++j)
+      if (k < d_in) {
+        h[L.w1 + k * kHid + j] = a1 * uni();
+        mask[L.w1 + k * kHid + j] = 1;
+      }
+  for (int j = 0; j < kHid; ++j) mask[L.b1 + j] = mask[L.b2 + j] = 1;
+  for (int k = 0; k < kHid * kHid; ++k) {
+    h[L.w2 + k] = a2 * uni();
+    mask[L.w2 + k] = 1;
+  }
+  for (int k = 0; k < kHid; ++k)
+    for (int j = 0; j < kCode; ++j) {
+      h[L.w3 + k * kOutP + j] = a2 * uni();
+      mask[L.w3 + k * kOutP + j] = 1;
+    }
+  for (int j = 0; j < kCode; ++j) mask[L.b3 + j] = 1;
+  int st = HDB_OK;
+  if ((st = mlp_alloc(ctx, reinterpret_cast<void**>(&net->prm), L.total * 4)) != HDB_OK ||
+      (st = mlp_alloc(ctx, reinterpret_cast<void**>(&net->grad), L.total * 4)) != HDB_OK ||
+      (st = mlp_alloc(ctx, reinterpret_cast<void**>(&net->m), L.total * 4)) != HDB_OK ||
+      (st = mlp_alloc(ctx, reinterpret_cast<void**>(&net->v), L.total * 4)) != HDB_OK ||
+      (st = mlp_alloc(ctx, reinterpret_cast<void**>(&net->mask), L.total)) != HDB_OK) {
+    hdb_mlp_destroy(net);
+    return st;
+  }
+  HDB_HIP(hipMemcpy(net->prm, h.data(), L.total * 4, hipMemcpyHostToDevice));
+  HDB_HIP(hipMemcpy(net->mask, mask.data(), L.total, hipMemcpyHostToDevice));
+  HDB_HIP(hipMemset(net->grad, 0, L.total * 4));
+  HDB_HIP(hipMemset(net->m, 0, L.total * 4));
+  HDB_HIP(hipMemset(net->v, 0, L.total * 4));
+  *out = net;
+  return HDB_OK;
+}
+
+int hdb_mlp_destroy(hdb_mlp* net) {
+  if (!net) return HDB_EINVAL;
+  (void)hipFree(net->prm);
+  (void)hipFree(net->grad);
+  (void)hipFree(net->m);
+  (void)hipFree(net->v);
+  (void)hipFree(net->mask);
+  delete net;
+  return HDB_OK;
+}
+
+int64_t hdb_mlp_param_count(const hdb_mlp* net) { return net ? net->L.total : -1; }
+
+int hdb_mlp_params(hdb_ctx* ctx, hdb_mlp* net, float* host_in, float* host_out) {
+  if (!ctx || !net) return HDB_EINVAL;
+  HDB_HIP(hipStreamSynchronize(ctx->stream));
+  if (host_in) HDB_HIP(hipMemcpy(net->prm, host_in, net->L.total * 4, hipMemcpyHostToDevice));
+  if (host_out) HDB_HIP(hipMemcpy(host_out, net->prm, net->L.total * 4, hipMemcpyDeviceToHost));
+  return HDB_OK;
+}
+
+int hdb_mlp_forward(hdb_ctx* ctx, hdb_mlp* net, const float* X, int64_t m, int32_t d, float* Z) {
+  if (!ctx || !net || !X || !Z || m < 0 || d != net->L.kin) return ctx ? ctx->fail(HDB_EINVAL, "bad forward args") : 1;
+  HDB_HIP(hipSetDevice(ctx->device));
+  if (m == 0) return HDB_OK;
+  return mlp_forward(ctx, net, X, d, nullptr, m, nullptr, nullptr, nullptr, nullptr, Z);
+}
+
+int hdb_mlp_loss_grad(hdb_ctx* ctx, hdb_mlp* net, const float* X, int64_t n, int32_t d, const int32_t* rows,
+                      int64_t batch, double* loss, float* grad_host) {
+  if (!ctx || !net || !X || !rows || batch < 1 || d != net->L.kin || n < 1)
+    return ctx ? ctx->fail(HDB_EINVAL, "bad loss_grad args") : 1;
+  HDB_HIP(hipSetDevice(ctx->device));
+  HDB_CALL(mlp_loss_grad(ctx, net, X, d, rows, batch, loss));
+  if (grad_host) {
+    HDB_HIP(hipMemcpyAsync(grad_host, net->grad, net->L.total * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HDB_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  return HDB_OK;
+}
+
+int hdb_mlp_train(hdb_ctx* ctx, hdb_mlp* net, const float* X, int64_t n, int32_t d, const hdb_nnd_element* lists,
+                  int32_t steps, int32_t batch, float lr, uint64_t seed, double* final_loss) {
+  if (!ctx || !net || !X || !lists || n < 2 || steps < 0 || batch < 1 || d != net->L.kin)
+    return ctx ? ctx->fail(HDB_EINVAL, "bad train args") : 1;
+  HDB_HIP(hipSetDevice(ctx->device));
+  Stage st(ctx, "mlp_train");
+  int32_t* rows;
+  HDB_CALL(scratch_t(ctx, kSlotAhKeys2, 3 * static_cast<int64_t>(batch), &rows));
+  for (int s = 0; s < steps; ++s) {
+    hipLaunchKernelGGL(k_mlp_batch, dim3(grid_for(batch, 256)), dim3(256), 0, ctx->stream, lists, n,
+                       static_cast<int64_t>(batch), seed + 0x100000001ull * static_cast<uint64_t>(net->step), rows);
+    HDB_LAUNCHED();
+    HDB_CALL(mlp_loss_grad(ctx, net, X, d, rows, batch, (s == steps - 1) ? final_loss : nullptr));
+    net->step += 1;
+    const float c1 = 1.0f - std::pow(0.9f, static_cast<float>(net->step));
+    const float c2 = 1.0f - std::pow(0.999f, static_cast<float>(net->step));
+    hipLaunchKernelGGL(k_adam, dim3(grid_for(net->L.total, 256)), dim3(256), 0, ctx->stream, net->prm, net->grad,
+                       net->m, net->v, net->L.total, lr, c1, c2, net->mask);
+    HDB_LAUNCHED();
+  }
+  return HDB_OK;
+}
+
+int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* net, const float* X, int64_t n, int32_t d, uint64_t seed,
+                     hdb_nnd_element* lists) {
+  if (!ctx || !net || !X || !lists || n < 34 || d != net->L.kin)
+    return ctx ? ctx->fail(HDB_EINVAL, "bad learned_init args (n must exceed 33)") : 1;
+  HDB_HIP(hipSetDevice(ctx->device));
+  Stage st(ctx, "learned_init");
+  float *Z, *u;
+  uint32_t *keys, *ids;
+  HDB_CALL(scratch_t(ctx, kSlotMisc0, n * kOutP, &Z));
+  HDB_CALL(scratch_t(ctx, kSlotMisc1, n, &keys));
+  HDB_CALL(scratch_t(ctx, kSlotMisc2, n, &ids));
+  HDB_CALL(scratch_t(ctx, kSlotBfTmp, d, &u));
+  std::vector<float> uh(d);
+  uint64_t s = seed * 0x2545F4914F6CDD1Dull + 7;
+  double nrm = 0.0;
+  for (int k = 0; k < d; ++k) {  // Box-Muller Gaussian direction
+    s = xorshift64star(s);
+    const double u1 = ((s >> 11) + 1.0) / 9007199254740993.0;
+    s = xorshift64star(s);
+    const double u2 = (s >> 11) / 9007199254740992.0;
+    uh[k] = static_cast<float>(std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2));
+    nrm += static_cast<double>(uh[k]) * uh[k];
+  }
+  for (int k = 0; k < d; ++k) uh[k] = static_cast<float>(uh[k] / std::sqrt(nrm));
+  HDB_HIP(hipMemcpyAsync(u, uh.data(), d * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  HDB_CALL(mlp_forward(ctx, net, X, d, nullptr, n, nullptr, nullptr, nullptr, nullptr, Z));
+  // x.u range: 4 standard deviations of a unit direction over typical data
+  hipLaunchKernelGGL(k_mlp_keys, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, Z, X, d, u, 64.0f, n, keys, ids);
+  HDB_LAUNCHED();
+  HDB_CALL(radix_sort_u32_u32(ctx, keys, ids, n, 0, 32));
+  hipLaunchKernelGGL(k_mlp_windows, dim3(grid_for(n * kK, 256)), dim3(256), 0, ctx->stream, ids, n, lists);
+  HDB_LAUNCHED();
+  return HDB_OK;
+}
+
+}  // extern "C"

@@ -53,6 +53,7 @@ struct NndBufs {
   float* worst;
   unsigned long long* part;  // per join block: [0] proposals, [1] capacity drops, [2] reverse drops, [3] candidates
   int join_keep;
+  int distance_mode;  // 0: FP32 join, 1: BF16 MFMA shortlist + exact FP32 re-rank
 };
 
 __device__ __forceinline__ hdb_nnd_element key_to_elem(uint64_t key) {
@@ -246,34 +247,15 @@ __global__ __launch_bounds__(256) void k_nnd_sample(const hdb_nnd_element* __res
 
 // ------------------------------------------------------------ local join --
 // Per point p (one wave): new = sort/dedup(fwd_new + rev_new), old =
-// sort/dedup(fwd_old + rev_old), new -= old (ShrinkGraph :405-510).
-// Candidate c (lane c) computes the squared distance to each partner -- the
-// other new candidates and the old ones if c is new (NEWNEW), only the old
-// ones if c is new in merge mode, the new ones if c is old -- and proposes
-// its nearest partners (GetMinElement/2/3 :852-961) to its own list unless
-// they cannot beat the list's current 32nd entry.  Proposals of the block
-// are compacted and appended with one atomic per block.
-template <int D, bool NEWNEW, int KEEP>
-__global__ __launch_bounds__(kJoinWaves * 64) void k_nnd_join(const float* __restrict__ X, int64_t n, int d, int S,
-                                                              NndBufs b) {
-  constexpr int WPB = kJoinWaves;
-  constexpr bool kOwnReg = (D > 0 && D <= 32);
-  constexpr int kStride = kOwnReg ? ((D + 3) & ~3) : 0;
-  constexpr int kSlots = KEEP == 0 ? 2 : KEEP;
-  extern __shared__ float lds_dyn[];
-  __shared__ uint32_t s_new[WPB][64];
-  __shared__ uint32_t s_old[WPB][64];
-  __shared__ uint32_t s_cand[WPB][64];
-  __shared__ int32_t s_wcnt[WPB];
-  __shared__ unsigned long long s_base;
+// sort/dedup(fwd_old + rev_old), new -= old (ShrinkGraph :405-510).  The
+// candidate ids land in s_cand[w] (new first); every wave of the block must
+// call this (it holds block barriers).
+template <int WPB>
+__device__ __forceinline__ void join_candidates(const NndBufs& b, int64_t p, bool live, int S, uint32_t (*s_new)[64],
+                                                uint32_t (*s_old)[64], uint32_t (*s_cand)[64], int* nn_out,
+                                                int* nc_out, int* rev_drop_out) {
   const unsigned lane = __lane_id();
   const unsigned w = threadIdx.x >> 6;
-  const int64_t p = static_cast<int64_t>(blockIdx.x) * WPB + w;
-  const bool live = p < n;
-  const int dd = D > 0 ? D : d;
-  const int stride = kOwnReg ? kStride : dd + 1;
-  float* vec = lds_dyn + static_cast<int64_t>(w) * 64 * stride;
-
   uint32_t v = 0xFFFFFFFFu;
   int rev_drop = 0;
   if (live) {
@@ -320,90 +302,31 @@ __global__ __launch_bounds__(kJoinWaves * 64) void k_nnd_join(const float* __res
   }
   const uint64_t bal2 = __ballot(keep_new);
   const int nn = __popcll(bal2);
-  const int nc = nn + no;
   if (keep_new) s_cand[w][__popcll(bal2 & lanemask_lt())] = x;
   if (lane < static_cast<unsigned>(no)) s_cand[w][nn + lane] = s_old[w][lane];
   __syncthreads();
+  *nn_out = nn;
+  *nc_out = nn + no;
+  *rev_drop_out = rev_drop;
+}
 
-  const bool active = lane < static_cast<unsigned>(nc);
-  const uint32_t my_id = active ? s_cand[w][lane] : 0u;
-  float own[kOwnReg ? kStride : 1];
-  if (kOwnReg) {
-    if (active) {
-      const float* src = X + static_cast<int64_t>(my_id) * D;
-#pragma unroll
-      for (int k = 0; k < D; ++k) own[k] = src[k];
-    } else {
-#pragma unroll
-      for (int k = 0; k < D; ++k) own[k] = 0.0f;
-    }
-#pragma unroll
-    for (int k = D; k < kStride; ++k) own[k] = 0.0f;
-    if (active) {
-#pragma unroll
-      for (int k = 0; k < kStride; ++k) vec[lane * kStride + k] = own[k];
-    }
-  } else {
-    for (int c = 0; c < nc; ++c) {
-      const float* src = X + static_cast<int64_t>(s_cand[w][c]) * dd;
-      for (int k = lane; k < dd; k += 64) vec[c * stride + k] = src[k];
-    }
-  }
-  __syncthreads();
-
-  // partner range of this lane
-  int jb, je;
-  if (static_cast<int>(lane) < nn) {
-    jb = NEWNEW ? 0 : nn;
-    je = nc;
-  } else {
-    jb = 0;
-    je = nn;
-  }
-  // KEEP == 0: the reference's two kernels -- a new candidate takes its
-  // nearest new partner (NewNeighborsCompareKernel) and its nearest old
-  // partner (TiledNewOldNeighborsCompareKernel), an old one its nearest new
-  // partner.  KEEP == T > 0: every candidate proposes its T nearest partners.
-  uint64_t top[kSlots];
-#pragma unroll
-  for (int s = 0; s < kSlots; ++s) top[s] = kKeyMax;
-  for (int j = 0; j < nc; ++j) {
-    const float* row = vec + j * stride;
-    float acc;
-    if (kOwnReg) {
-      acc = 0.0f;
-#pragma unroll
-      for (int k = 0; k < D; ++k) {
-        const float diff = own[k] - row[k];
-        acc = __builtin_fmaf(diff, diff, acc);
-      }
-    } else {
-      acc = d2_any(vec + lane * stride, row, dd);
-    }
-    const bool ok = active && j >= jb && j < je && j != static_cast<int>(lane);
-    uint64_t key = ok ? make_key(acc, s_cand[w][j], 1u) : kKeyMax;
-    if (KEEP == 0) {
-      if (j < nn) top[0] = key < top[0] ? key : top[0];
-      else top[1] = key < top[1] ? key : top[1];
-    } else {
-#pragma unroll
-      for (int s = 0; s < kSlots; ++s) {
-        const uint64_t a = top[s];
-        const bool sw = key < a;
-        top[s] = sw ? key : a;
-        key = sw ? a : key;
-      }
-    }
-  }
+// Filters each lane's proposals against its list bound, compacts the block's
+// proposals with one atomic and appends them; accumulates the block's
+// counters.  Every wave of the block must call this.
+template <int WPB, int SLOTS>
+__device__ __forceinline__ void join_emit(const NndBufs& b, uint64_t (&top)[SLOTS], bool active, uint32_t my_id,
+                                          int nc, int rev_drop, int32_t* s_wcnt, uint32_t (*s_new)[64],
+                                          unsigned long long* s_base) {
+  const unsigned lane = __lane_id();
+  const unsigned w = threadIdx.x >> 6;
   const float lim = active ? b.worst[my_id] : 0.0f;
   int cnt = 0;
 #pragma unroll
-  for (int s = 0; s < kSlots; ++s) {
+  for (int s = 0; s < SLOTS; ++s) {
     const bool ok = active && top[s] != kKeyMax && key_d2(top[s]) < lim;
     if (!ok) top[s] = kKeyMax;
     cnt += ok ? 1 : 0;
   }
-  // compaction: exclusive prefix of lane counts, wave offsets, one atomic
   int incl = cnt;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
@@ -418,14 +341,14 @@ __global__ __launch_bounds__(kJoinWaves * 64) void k_nnd_join(const float* __res
     if (q < static_cast<int>(w)) woff += s_wcnt[q];
     total += s_wcnt[q];
   }
-  if (threadIdx.x == 0) s_base = total ? atomicAdd(b.prop_cnt, static_cast<unsigned long long>(total)) : 0ull;
+  if (threadIdx.x == 0) *s_base = total ? atomicAdd(b.prop_cnt, static_cast<unsigned long long>(total)) : 0ull;
   __syncthreads();
-  const int64_t base = static_cast<int64_t>(s_base);
+  const int64_t base = static_cast<int64_t>(*s_base);
   const bool fits = base + total <= b.prop_cap;
   if (fits) {
     int64_t o = base + woff + incl - cnt;
 #pragma unroll
-    for (int s = 0; s < kSlots; ++s) {
+    for (int s = 0; s < SLOTS; ++s) {
       if (top[s] != kKeyMax) {
         b.prop_tgt[o] = my_id;
         b.prop_key[o] = top[s];
@@ -458,6 +381,226 @@ __global__ __launch_bounds__(kJoinWaves * 64) void k_nnd_join(const float* __res
     b.part[4 * static_cast<int64_t>(blockIdx.x) + 2] += static_cast<unsigned long long>(tot);
     b.part[4 * static_cast<int64_t>(blockIdx.x) + 3] += static_cast<unsigned long long>(cand);
   }
+}
+
+// Keeps the SLOTS smallest keys (ascending) of a stream.
+template <int SLOTS>
+__device__ __forceinline__ void top_insert(uint64_t (&top)[SLOTS], uint64_t key) {
+#pragma unroll
+  for (int s = 0; s < SLOTS; ++s) {
+    const uint64_t a = top[s];
+    const bool sw = key < a;
+    top[s] = sw ? key : a;
+    key = sw ? a : key;
+  }
+}
+
+// FP32 join.  Candidate c (lane c) computes the squared distance to each
+// partner -- the other new candidates and the old ones if c is new (NEWNEW),
+// only the old ones if c is new in merge mode, the new ones if c is old --
+// and proposes its nearest partners (GetMinElement/2/3 :852-961) to its own
+// list unless they cannot beat the list's current 32nd entry.
+template <int D, bool NEWNEW, int KEEP>
+__global__ __launch_bounds__(kJoinWaves * 64) void k_nnd_join(const float* __restrict__ X, int64_t n, int d, int S,
+                                                              NndBufs b) {
+  constexpr int WPB = kJoinWaves;
+  constexpr bool kOwnReg = (D > 0 && D <= 32);
+  constexpr int kStride = kOwnReg ? ((D + 3) & ~3) : 0;
+  constexpr int kSlots = KEEP == 0 ? 2 : KEEP;
+  extern __shared__ float lds_dyn[];
+  __shared__ uint32_t s_new[WPB][64];
+  __shared__ uint32_t s_old[WPB][64];
+  __shared__ uint32_t s_cand[WPB][64];
+  __shared__ int32_t s_wcnt[WPB];
+  __shared__ unsigned long long s_base;
+  const unsigned lane = __lane_id();
+  const unsigned w = threadIdx.x >> 6;
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * WPB + w;
+  const int dd = D > 0 ? D : d;
+  const int stride = kOwnReg ? kStride : dd + 1;
+  float* vec = lds_dyn + static_cast<int64_t>(w) * 64 * stride;
+
+  int nn, nc, rev_drop;
+  join_candidates<WPB>(b, p, p < n, S, s_new, s_old, s_cand, &nn, &nc, &rev_drop);
+
+  const bool active = lane < static_cast<unsigned>(nc);
+  const uint32_t my_id = active ? s_cand[w][lane] : 0u;
+  float own[kOwnReg ? kStride : 1];
+  if (kOwnReg) {
+    if (active) {
+      const float* src = X + static_cast<int64_t>(my_id) * D;
+#pragma unroll
+      for (int k = 0; k < D; ++k) own[k] = src[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < D; ++k) own[k] = 0.0f;
+    }
+#pragma unroll
+    for (int k = D; k < kStride; ++k) own[k] = 0.0f;
+    if (active) {
+#pragma unroll
+      for (int k = 0; k < kStride; ++k) vec[lane * kStride + k] = own[k];
+    }
+  } else {
+    for (int c = 0; c < nc; ++c) {
+      const float* src = X + static_cast<int64_t>(s_cand[w][c]) * dd;
+      for (int k = lane; k < dd; k += 64) vec[c * stride + k] = src[k];
+    }
+  }
+  __syncthreads();
+
+  int jb, je;  // partner range of this lane
+  if (static_cast<int>(lane) < nn) {
+    jb = NEWNEW ? 0 : nn;
+    je = nc;
+  } else {
+    jb = 0;
+    je = nn;
+  }
+  // KEEP == 0: the reference's two kernels -- a new candidate takes its
+  // nearest new partner (NewNeighborsCompareKernel) and its nearest old
+  // partner (TiledNewOldNeighborsCompareKernel), an old one its nearest new
+  // partner.  KEEP == T > 0: every candidate proposes its T nearest partners.
+  uint64_t top[kSlots];
+#pragma unroll
+  for (int s = 0; s < kSlots; ++s) top[s] = kKeyMax;
+  for (int j = 0; j < nc; ++j) {
+    const float* row = vec + j * stride;
+    float acc;
+    if (kOwnReg) {
+      acc = 0.0f;
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        const float diff = own[k] - row[k];
+        acc = __builtin_fmaf(diff, diff, acc);
+      }
+    } else {
+      acc = d2_any(vec + lane * stride, row, dd);
+    }
+    const bool ok = active && j >= jb && j < je && j != static_cast<int>(lane);
+    const uint64_t key = ok ? make_key(acc, s_cand[w][j], 1u) : kKeyMax;
+    if (KEEP == 0) {
+      if (j < nn) top[0] = key < top[0] ? key : top[0];
+      else top[1] = key < top[1] ? key : top[1];
+    } else {
+      top_insert<kSlots>(top, key);
+    }
+  }
+  join_emit<WPB, kSlots>(b, top, active, my_id, nc, rev_drop, s_wcnt, s_new, &s_base);
+}
+
+// BF16 join (distance_mode 1): the 64 x 64 Gram matrix of the candidates is
+// computed with v_mfma_f32_16x16x32_bf16 (BF16 inputs, FP32 accumulation)
+// from the FP32 rows staged in LDS; approximate squared distances
+// |x|^2 + |y|^2 - 2 x.y pick each candidate's SHORT nearest partners, which
+// are re-ranked with the exact FP32 formula, so proposals -- and therefore
+// lists, core distances and MST weights -- carry exact FP32 distances only.
+// Lane layout of the 16x16x32 instruction: A/B lane l holds row (l % 16),
+// k = 8*(l/16) .. +7; the result lane l holds rows 4*(l/16) + q, column l % 16.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+template <bool NEWNEW, int KEEP>
+__global__ __launch_bounds__(64) void k_nnd_join_mfma(const float* __restrict__ X, int64_t n, int d, int dp, int S,
+                                                      NndBufs b) {
+  constexpr int WPB = 1;
+  constexpr int kSlots = KEEP == 0 ? 2 : KEEP;
+  constexpr int kShort = kSlots + 2;
+  extern __shared__ float lds_dyn[];
+  __shared__ uint32_t s_new[WPB][64];
+  __shared__ uint32_t s_old[WPB][64];
+  __shared__ uint32_t s_cand[WPB][64];
+  __shared__ int32_t s_wcnt[WPB];
+  __shared__ unsigned long long s_base;
+  __shared__ float s_norm[64];
+  const unsigned lane = __lane_id();
+  const int64_t p = blockIdx.x;
+  const int rs = dp + 4;                 // fp32 row stride (16-byte aligned rows)
+  float* vec = lds_dyn;                  // 64 x rs
+  float* gram = lds_dyn + 64 * rs;       // 64 x 65
+
+  int nn, nc, rev_drop;
+  join_candidates<WPB>(b, p, p < n, S, s_new, s_old, s_cand, &nn, &nc, &rev_drop);
+
+  // stage the FP32 rows (zero rows past nc and zero columns past d)
+  for (int c = 0; c < 64; ++c) {
+    const bool row_ok = c < nc;
+    const float* src = row_ok ? X + static_cast<int64_t>(s_cand[0][c]) * d : X;
+    for (int k = lane; k < dp; k += 64) vec[c * rs + k] = (row_ok && k < d) ? src[k] : 0.0f;
+  }
+  __syncthreads();
+  {
+    float nrm = 0.0f;
+    for (int k = 0; k < dp; ++k) nrm = __builtin_fmaf(vec[lane * rs + k], vec[lane * rs + k], nrm);
+    s_norm[lane] = nrm;
+  }
+  // Gram tiles (I, J), J >= I, over the live 16-row blocks
+  const int tiles = (nc + 15) >> 4;
+  const int r16 = lane & 15, kq = lane >> 4;
+  for (int I = 0; I < tiles; ++I) {
+    for (int J = I; J < tiles; ++J) {
+      f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+      for (int k0 = 0; k0 < dp; k0 += 32) {
+        const float* ra = vec + (I * 16 + r16) * rs + k0 + 8 * kq;
+        const float* rb = vec + (J * 16 + r16) * rs + k0 + 8 * kq;
+        bf16x8_t fa, fb;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          fa[e] = static_cast<__bf16>(ra[e]);
+          fb[e] = static_cast<__bf16>(rb[e]);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int gi = I * 16 + 4 * kq + q, gj = J * 16 + r16;
+        gram[gi * 65 + gj] = acc[q];
+        gram[gj * 65 + gi] = acc[q];
+      }
+    }
+  }
+  __syncthreads();
+
+  const bool active = lane < static_cast<unsigned>(nc);
+  const uint32_t my_id = active ? s_cand[0][lane] : 0u;
+  int jb, je;
+  if (static_cast<int>(lane) < nn) {
+    jb = NEWNEW ? 0 : nn;
+    je = nc;
+  } else {
+    jb = 0;
+    je = nn;
+  }
+  // approximate shortlist: key (approx d2 bits, partner index j)
+  uint64_t sl[kShort];
+#pragma unroll
+  for (int s = 0; s < kShort; ++s) sl[s] = kKeyMax;
+  const float ni = s_norm[lane];
+  for (int j = 0; j < nc; ++j) {
+    const bool ok = active && j >= jb && j < je && j != static_cast<int>(lane);
+    float a = ni + s_norm[j] - 2.0f * gram[lane * 65 + j];
+    a = a > 0.0f ? a : 0.0f;
+    const uint64_t key = ok ? ((static_cast<uint64_t>(__float_as_uint(a)) << 32) | static_cast<uint32_t>(j)) : kKeyMax;
+    top_insert<kShort>(sl, key);
+  }
+  // exact FP32 re-rank of the shortlist
+  uint64_t top[kSlots];
+#pragma unroll
+  for (int s = 0; s < kSlots; ++s) top[s] = kKeyMax;
+#pragma unroll
+  for (int s = 0; s < kShort; ++s) {
+    if (sl[s] == kKeyMax) continue;
+    const int j = static_cast<int>(static_cast<uint32_t>(sl[s]));
+    const float e = d2_any(vec + lane * rs, vec + j * rs, d);
+    const uint64_t key = make_key(e, s_cand[0][j], 1u);
+    if (KEEP == 0) {
+      if (j < nn) top[0] = key < top[0] ? key : top[0];
+      else top[1] = key < top[1] ? key : top[1];
+    } else {
+      top_insert<kSlots>(top, key);
+    }
+  }
+  join_emit<WPB, kSlots>(b, top, active, my_id, nc, rev_drop, s_wcnt, s_new, &s_base);
 }
 
 // ----------------------------------------------------------- list update --
@@ -614,16 +757,17 @@ static int get_bufs(hdb_ctx* ctx, int64_t n, const hdb_params& prm, NndBufs* b) 
   unsigned long long* st;
   HDB_CALL(scratch_t(ctx, kSlotStats, 8, &st));
   b->prop_cnt = st;
-  const int64_t blocks = (n + kJoinWaves - 1) / kJoinWaves;
+  const int64_t blocks = n;  // join grids use at most n blocks (one per point in BF16 mode)
   HDB_CALL(scratch_t(ctx, kSlotPart, 4 * blocks, &b->part));
   HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(b->part), 4 * blocks, 0));
   HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(st), 8, 0));
   b->join_keep = prm.join_keep;
+  b->distance_mode = prm.distance_mode;
   return HDB_OK;
 }
 
 static int read_stats(hdb_ctx* ctx, int64_t n, const NndBufs& b) {
-  const int64_t blocks = (n + kJoinWaves - 1) / kJoinWaves;
+  const int64_t blocks = n;  // join grids use at most n blocks (one per point in BF16 mode)
   hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, ctx->stream, b.part, blocks, b.prop_cnt + 4);
   HDB_LAUNCHED();
   unsigned long long s[4];
@@ -655,8 +799,27 @@ static int launch_join_t(hdb_ctx* ctx, const float* X, int64_t n, int d, int S, 
   }
 }
 
+template <bool NEWNEW, int KEEP>
+static int launch_join_mfma_k(hdb_ctx* ctx, const float* X, int64_t n, int d, int S, const NndBufs& b) {
+  const int dp = (d + 31) & ~31;
+  const size_t lds = (static_cast<size_t>(64) * (dp + 4) + 64 * 65) * sizeof(float);
+  hipLaunchKernelGGL((k_nnd_join_mfma<NEWNEW, KEEP>), dim3(static_cast<unsigned>(n)), dim3(64), lds, ctx->stream, X,
+                     n, d, dp, S, b);
+  HDB_LAUNCHED();
+  return HDB_OK;
+}
+
 template <bool NEWNEW>
 static int launch_join(hdb_ctx* ctx, const float* X, int64_t n, int d, int S, const NndBufs& b) {
+  if (b.distance_mode == 1) {
+    switch (b.join_keep) {
+      case 0: return launch_join_mfma_k<NEWNEW, 0>(ctx, X, n, d, S, b);
+      case 1: return launch_join_mfma_k<NEWNEW, 1>(ctx, X, n, d, S, b);
+      case 2: return launch_join_mfma_k<NEWNEW, 2>(ctx, X, n, d, S, b);
+      case 3: return launch_join_mfma_k<NEWNEW, 3>(ctx, X, n, d, S, b);
+      default: return launch_join_mfma_k<NEWNEW, 4>(ctx, X, n, d, S, b);
+    }
+  }
   if (d == 12) return launch_join_t<12, NEWNEW>(ctx, X, n, d, S, b);
   if (d == 32) return launch_join_t<32, NEWNEW>(ctx, X, n, d, S, b);
   return launch_join_t<0, NEWNEW>(ctx, X, n, d, S, b);
@@ -713,6 +876,8 @@ static int check_params(hdb_ctx* ctx, const hdb_params& prm, int64_t n, int d) {
   if (prm.bucket_cap < 8 || prm.bucket_cap > 1024) return ctx->fail(HDB_EINVAL, "bucket_cap must be in [8,1024]");
   if (prm.nnd_iteration < 0 || prm.merge_iteration < 0) return ctx->fail(HDB_EINVAL, "negative iteration count");
   if (prm.join_keep < 0 || prm.join_keep > 4) return ctx->fail(HDB_EINVAL, "join_keep must be in [0,4]");
+  if (prm.distance_mode < 0 || prm.distance_mode > 1) return ctx->fail(HDB_EINVAL, "distance_mode must be 0 or 1");
+  if (prm.distance_mode == 1 && d > 160) return ctx->fail(HDB_EUNSUPPORTED, "BF16 join built for d <= 160");
   if (n < 2) return ctx->fail(HDB_EINVAL, "need at least 2 points");
   if (n >= (static_cast<int64_t>(1) << 30)) return ctx->fail(HDB_EINVAL, "n must be below 2^30");
   if (d < 1) return ctx->fail(HDB_EINVAL, "d must be >= 1");

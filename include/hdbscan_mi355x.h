@@ -204,6 +204,36 @@ int hdb_pipeline(hdb_ctx* ctx, const float* X_dev, int64_t n, int32_t d,
                  hdb_mst_edge* edges_dev, int64_t* m_out, int64_t* components_out,
                  int64_t* csr_edges_out);
 
+/* ---- learned initialiser ------------------------------------------------------
+ * No reference counterpart: replaces the random index init of
+ * InitKNNGraphIndexKernel (gpuknn/nndescent.cu:1673-1692).  An MLP
+ * d_in -> 128 -> 128 -> 12 (GELU; BF16 MFMA products, FP32 accumulation and
+ * FP32 master weights) maps points to 12 sign bits = 4096 buckets; each
+ * point's initial list is its 32 neighbours in the (bucket, x.u) order.
+ * Parameter vector layout (FP32): W1[kinp][128], b1[128], W2[128][128],
+ * b2[128], W3[128][16], b3[16] with kinp = d_in rounded up to 32; padding
+ * entries stay zero. */
+typedef struct hdb_mlp hdb_mlp;
+int hdb_mlp_create(hdb_ctx* ctx, int32_t d_in, uint64_t seed, hdb_mlp** out);
+int hdb_mlp_destroy(hdb_mlp* net);
+int64_t hdb_mlp_param_count(const hdb_mlp* net);
+/* copy parameters in (host_in != NULL) and/or out (host_out != NULL) */
+int hdb_mlp_params(hdb_ctx* ctx, hdb_mlp* net, float* host_in, float* host_out);
+/* Z_dev: m x 16 (columns 12..15 are zero padding) */
+int hdb_mlp_forward(hdb_ctx* ctx, hdb_mlp* net, const float* X_dev, int64_t m, int32_t d, float* Z_dev);
+/* Loss and gradient for `batch` triples: rows_dev holds 3*batch point ids
+ * (anchors, positives, negatives).  *loss (optional) and grad_host
+ * (optional, param_count floats) receive the results. */
+int hdb_mlp_loss_grad(hdb_ctx* ctx, hdb_mlp* net, const float* X_dev, int64_t n, int32_t d,
+                      const int32_t* rows_dev, int64_t batch, double* loss, float* grad_host);
+/* Adam steps on triples drawn from lists_dev (the kNN graph of X_dev). */
+int hdb_mlp_train(hdb_ctx* ctx, hdb_mlp* net, const float* X_dev, int64_t n, int32_t d,
+                  const hdb_nnd_element* lists_dev, int32_t steps, int32_t batch, float lr,
+                  uint64_t seed, double* final_loss);
+/* Initial lists (ids only; distances are computed by hdb_nndescent_from). */
+int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* net, const float* X_dev, int64_t n, int32_t d,
+                     uint64_t seed, hdb_nnd_element* lists_dev);
+
 /* ---- host labels (HDBSCAN/trees/tree.cu) ----------------------------------- */
 /* Single linkage -> condensed tree (min cluster size mpts) -> stability ->
  * EOM selection -> labels, following tree.cu:171-746 on host memory.
