@@ -1,0 +1,139 @@
+"""Learned initialiser for NN-Descent (no reference counterpart).
+
+LearnedInit.fit() runs plain NN-Descent on a seeded subsample, then trains
+the MLP on the GPU (hdb_mlp_train: BF16 MFMA forward/backward, Adam).
+nndescent_learned() seeds every list with the point's 32 neighbours in the
+(bucket code, x.u) order, runs NN-Descent from there, checks recall@32 on
+sampled queries against an exact GPU brute force, and falls back to plain
+random-init NN-Descent when the check fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import api
+from ._lib import NND_ELEMENT
+
+
+class LearnedInit:
+    def __init__(self, ctx: api.Context, d: int, seed: int = 0):
+        self.ctx = ctx
+        self.d = int(d)
+        h = C.c_void_p()
+        ctx._check(ctx.lib.hdb_mlp_create(ctx.handle, self.d, C.c_uint64(seed), C.byref(h)))
+        self.handle = h.value
+        self.train_seconds = 0.0
+        self.final_loss = float("nan")
+
+    def close(self):
+        if self.handle:
+            self.ctx.lib.hdb_mlp_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def param_count(self) -> int:
+        return int(self.ctx.lib.hdb_mlp_param_count(self.handle))
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(self.param_count, np.float32)
+        self.ctx._check(self.ctx.lib.hdb_mlp_params(self.ctx.handle, self.handle, None, out.ctypes.data))
+        return out
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.param_count
+        self.ctx._check(self.ctx.lib.hdb_mlp_params(self.ctx.handle, self.handle, p.ctypes.data, None))
+
+    def forward(self, X) -> np.ndarray:
+        Xd = self.ctx.dev(X, np.float32)
+        Z = self.ctx.empty((Xd.shape[0], 16), np.float32)
+        self.ctx._check(self.ctx.lib.hdb_mlp_forward(self.ctx.handle, self.handle, Xd.ptr, Xd.shape[0], self.d, Z.ptr))
+        return Z.numpy()[:, :12]
+
+    def loss_grad(self, X, rows, batch):
+        Xd = self.ctx.dev(X, np.float32)
+        r = self.ctx.dev(np.ascontiguousarray(rows, np.int32))
+        loss = C.c_double()
+        g = np.empty(self.param_count, np.float32)
+        self.ctx._check(self.ctx.lib.hdb_mlp_loss_grad(self.ctx.handle, self.handle, Xd.ptr, Xd.shape[0], self.d,
+                                                       r.ptr, int(batch), C.byref(loss), g.ctypes.data))
+        return loss.value, g
+
+    def train(self, Xsub, lists_sub, steps: int, batch: int = 4096, lr: float = 1e-3, seed: int = 0) -> float:
+        Xd = self.ctx.dev(Xsub, np.float32)
+        Ld = lists_sub if isinstance(lists_sub, api.DeviceArray) else self.ctx.to_device(
+            np.ascontiguousarray(lists_sub, NND_ELEMENT))
+        loss = C.c_double()
+        self.ctx._check(self.ctx.lib.hdb_mlp_train(self.ctx.handle, self.handle, Xd.ptr, Xd.shape[0], self.d, Ld.ptr,
+                                                   int(steps), int(batch), float(lr), C.c_uint64(seed),
+                                                   C.byref(loss)))
+        self.final_loss = loss.value
+        return loss.value
+
+    def fit(self, X: np.ndarray, n_sub: int = 200_000, nnd_iterations: int = 6, epochs: float = 2.0,
+            batch: int = 4096, lr: float = 2e-3, seed: int = 0) -> "LearnedInit":
+        """Plain NN-Descent on a seeded subsample, then GPU training."""
+        rng = np.random.default_rng(seed)
+        n = X.shape[0]
+        sub = np.sort(rng.choice(n, size=min(n_sub, n), replace=False))
+        Xs = np.ascontiguousarray(X[sub], np.float32)
+        t0 = time.perf_counter()
+        Xsd = self.ctx.to_device(Xs)
+        Ls = api.NNDescent(self.ctx, Xsd, iteration=nnd_iterations, store_result_in_device=True)
+        steps = max(1, int(epochs * len(sub) / batch))
+        self.train(Xsd, Ls, steps, batch, lr, seed)
+        self.ctx.synchronize()
+        self.train_seconds = time.perf_counter() - t0
+        return self
+
+    def init_lists(self, X, seed: int = 0) -> api.DeviceArray:
+        Xd = self.ctx.dev(X, np.float32)
+        n = Xd.shape[0]
+        L = self.ctx.empty((n, 32), NND_ELEMENT)
+        self.ctx._check(self.ctx.lib.hdb_learned_init(self.ctx.handle, self.handle, Xd.ptr, n, self.d,
+                                                      C.c_uint64(seed), L.ptr))
+        return L
+
+
+@dataclass
+class LearnedResult:
+    lists: api.DeviceArray
+    iterations: int
+    sampled_recall: float
+    fell_back: bool
+
+
+def sampled_recall(ctx: api.Context, Xd: api.DeviceArray, lists: api.DeviceArray, queries: int = 1000,
+                   seed: int = 99) -> float:
+    """recall@32 of the lists on sampled queries against the exact GPU brute force."""
+    n = Xd.shape[0]
+    q = np.random.default_rng(seed).choice(n, size=min(queries, n), replace=False).astype(np.int32)
+    gt, _ = api.bruteforce_knn(ctx, Xd, q, 32)
+    lab = lists.numpy()["label"][q]
+    hit = sum(len(np.intersect1d(lab[i], gt[i])) for i in range(len(q)))
+    return hit / (len(q) * 32.0)
+
+
+def nndescent_learned(ctx: api.Context, X, model: LearnedInit, iterations: int, target: float = 0.95,
+                      queries: int = 1000, params=None, fallback_iterations: int = 6) -> LearnedResult:
+    """NN-Descent from learned initial lists; falls back to random init when
+    the sampled recall@32 is below target."""
+    Xd = ctx.dev(X, np.float32)
+    p = params or api.default_params()
+    L = model.init_lists(Xd)
+    api.NNDescentFrom(ctx, Xd, L, iterations, params=p, store_result_in_device=True)
+    r = sampled_recall(ctx, Xd, L, queries)
+    if r >= target:
+        return LearnedResult(L, iterations, r, False)
+    Lf = api.NNDescent(ctx, Xd, iteration=fallback_iterations, params=p, store_result_in_device=True)
+    return LearnedResult(Lf, fallback_iterations, sampled_recall(ctx, Xd, Lf, queries), True)

@@ -1,0 +1,122 @@
+"""FP64 oracle of the learned-initialiser MLP (TEST INFRASTRUCTURE).
+
+Forward d_in -> 128 -> 128 -> 12 with exact-erf GELU, the triple loss
+  L = mean_b[ |t_a - t_p|^2 / 12 + (t_a . t_n / 12)^2 ] + sum_bits mean_rows(t)^2,
+t = tanh(z), and its analytic gradient, in float64.  With bf16=True every
+matrix-product operand is first rounded to BF16 (round to nearest even), as
+the GPU kernels do when they form MFMA fragments; sums stay in float64.  The
+parameter vector follows the C ABI layout: W1[kinp][128], b1[128],
+W2[128][128], b2[128], W3[128][16], b3[16].
+
+Only tests/ may import this module.  The gradient is pinned against central
+finite differences of the loss (bf16=False).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+H = 128
+CODE = 12
+OUTP = 16
+
+_erf = np.vectorize(math.erf, otypes=[np.float64])
+
+
+def layout(d_in: int) -> dict:
+    kinp = (d_in + 31) // 32 * 32
+    off = {}
+    o = 0
+    for name, size in (("w1", kinp * H), ("b1", H), ("w2", H * H), ("b2", H), ("w3", H * OUTP), ("b3", OUTP)):
+        off[name] = (o, size)
+        o += size
+    off["total"] = o
+    off["kinp"] = kinp
+    return off
+
+
+def unpack(p: np.ndarray, d_in: int):
+    L = layout(d_in)
+    kinp = L["kinp"]
+
+    def g(name, shape):
+        o, s = L[name]
+        return p[o:o + s].astype(np.float64).reshape(shape)
+
+    return (g("w1", (kinp, H))[:d_in], g("b1", (H,)), g("w2", (H, H)), g("b2", (H,)),
+            g("w3", (H, OUTP))[:, :CODE], g("b3", (OUTP,))[:CODE])
+
+
+def bf16(x: np.ndarray) -> np.ndarray:
+    """Round float32 values to the nearest BF16 (ties to even), as float64."""
+    f = np.ascontiguousarray(x, dtype=np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    return r.view(np.float32).astype(np.float64)
+
+
+def gelu(x):
+    return 0.5 * x * (1.0 + _erf(x / math.sqrt(2.0)))
+
+
+def gelu_grad(x):
+    return 0.5 * (1.0 + _erf(x / math.sqrt(2.0))) + x * np.exp(-0.5 * x * x) / math.sqrt(2.0 * math.pi)
+
+
+def forward(p, X, d_in, use_bf16=True):
+    W1, b1, W2, b2, W3, b3 = unpack(p, d_in)
+    r = bf16 if use_bf16 else (lambda a: np.asarray(a, np.float64))
+    P1 = r(X) @ r(W1) + b1
+    H1 = gelu(P1)
+    P2 = r(H1) @ r(W2) + b2
+    H2 = gelu(P2)
+    Z = r(H2) @ r(W3) + b3
+    return P1, H1, P2, H2, Z
+
+
+def loss_grad(p, X, rows, B, d_in, use_bf16=True):
+    """Loss and gradient (parameter-vector layout) for triples rows[0:B],
+    rows[B:2B], rows[2B:3B] of X."""
+    W1, b1, W2, b2, W3, b3 = unpack(p, d_in)
+    r = bf16 if use_bf16 else (lambda a: np.asarray(a, np.float64))
+    Xr = np.asarray(X, np.float32)[rows]
+    P1, H1, P2, H2, Z = forward(p, Xr, d_in, use_bf16)
+    t = np.tanh(Z)
+    R = 3 * B
+    ta, tp, tn = t[:B], t[B:2 * B], t[2 * B:]
+    mean = t.mean(axis=0)
+    s = (ta * tn).sum(1) / CODE
+    loss = np.mean(((ta - tp) ** 2).sum(1) / CODE + s * s) + (mean ** 2).sum()
+    dt = np.zeros_like(t)
+    dt[:B] = (2.0 / CODE) * (ta - tp) / B + (2.0 * s[:, None] / CODE) * tn / B
+    dt[B:2 * B] = -(2.0 / CODE) * (ta - tp) / B
+    dt[2 * B:] = (2.0 * s[:, None] / CODE) * ta / B
+    dt += 2.0 * mean / R
+    dZ = dt * (1.0 - t * t)
+    dW3 = r(H2).T @ r(dZ)
+    db3 = dZ.sum(0)
+    dH2 = r(dZ) @ r(W3).T
+    dP2 = dH2 * gelu_grad(P2)
+    dW2 = r(H1).T @ r(dP2)
+    db2 = dP2.sum(0)
+    dH1 = r(dP2) @ r(W2).T
+    dP1 = dH1 * gelu_grad(P1)
+    dW1 = r(Xr).T @ r(dP1)
+    db1 = dP1.sum(0)
+    L = layout(d_in)
+    g = np.zeros(L["total"], np.float64)
+
+    def put(name, val, full_shape):
+        o, sz = L[name]
+        buf = np.zeros(full_shape)
+        buf[tuple(slice(0, n) for n in val.shape)] = val
+        g[o:o + sz] = buf.ravel()
+
+    put("w1", dW1, (L["kinp"], H))
+    put("b1", db1, (H,))
+    put("w2", dW2, (H, H))
+    put("b2", db2, (H,))
+    put("w3", dW3, (H, OUTP))
+    put("b3", db3, (OUTP,))
+    return loss, g

@@ -1,0 +1,60 @@
+"""Learned initialiser on the GPU: the hand-written BF16 MFMA forward and
+backward kernels against the FP64 oracle (oracle/mlp_oracle.py), training
+lowers the loss, and the trained network seeds lists that beat random init."""
+import numpy as np
+import pytest
+
+from gpu_hdbscan_amd import api, datasets, learned
+from oracle import mlp_oracle as M
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def test_forward_and_gradient_match_fp64_oracle(ctx):
+    print("case: mlp forward/gradient vs FP64 oracle", flush=True)
+    rng = np.random.default_rng(3)
+    d = 32
+    X = rng.standard_normal((2000, d)).astype(np.float32) * 2
+    net = learned.LearnedInit(ctx, d, seed=5)
+    p = net.get_params()
+    Z = net.forward(X[:300])
+    Zr = M.forward(p.astype(np.float64), X[:300], d, use_bf16=True)[4]
+    assert np.max(np.abs(Z - Zr)) / np.max(np.abs(Zr)) < 2e-2
+    B = 512
+    rows = rng.integers(0, 2000, 3 * B).astype(np.int32)
+    loss, g = net.loss_grad(X, rows, B)
+    rloss, rg = M.loss_grad(p.astype(np.float64), X, rows, B, d, use_bf16=True)
+    print("loss", loss, rloss, flush=True)
+    assert abs(loss - rloss) <= 1e-3 * abs(rloss) + 1e-6
+    L = M.layout(d)
+    for name in ("w1", "b1", "w2", "b2", "w3", "b3"):
+        o, s = L[name]
+        a, b = g[o:o + s].astype(np.float64), rg[o:o + s]
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
+        print(name, "rel err", rel, flush=True)
+        assert rel < 3e-2, name
+    # against the unrounded FP64 model the BF16 error is bounded, too
+    _, rg64 = M.loss_grad(p.astype(np.float64), X, rows, B, d, use_bf16=False)
+    assert np.linalg.norm(g - rg64) / np.linalg.norm(rg64) < 0.1
+
+
+def test_training_and_learned_init(ctx):
+    print("case: learned init aniso200 60000x32", flush=True)
+    X = datasets.make("aniso200_5m_d32", n=60000, seed=4)
+    net = learned.LearnedInit(ctx, 32, seed=1)
+    Xd = ctx.to_device(X)
+    Lr = api.NNDescent(ctx, Xd, iteration=0, store_result_in_device=True)
+    q = np.arange(0, 60000, 60, dtype=np.int32)
+    gt, _ = O.bruteforce_fp64(X, q, 32)
+    r_random = O.recall(Lr.numpy()["label"][q], gt, 32)
+    L0 = net.init_lists(Xd)
+    r_untrained = O.recall(L0.numpy()["label"][q], gt, 32)
+    net.fit(X, n_sub=30000, epochs=4, batch=2048)
+    print("final loss", net.final_loss, "train s", net.train_seconds, flush=True)
+    L1 = net.init_lists(Xd)
+    r_trained = O.recall(L1.numpy()["label"][q], gt, 32)
+    print("initial-list recall random", r_random, "untrained", r_untrained, "trained", r_trained, flush=True)
+    assert r_trained > r_random
+    res = learned.nndescent_learned(ctx, Xd, net, iterations=4, target=0.9)
+    print("learned 4 its sampled recall", res.sampled_recall, "fallback", res.fell_back, flush=True)

@@ -1,0 +1,49 @@
+"""Pins of the MLP gradient oracle: analytic FP64 gradient against central
+finite differences of the loss, and the BF16 rounding helper against
+closed-form cases."""
+import numpy as np
+
+from oracle import mlp_oracle as M
+
+
+def test_bf16_rounding_closed_form():
+    x = np.array([1.0, 1.0 + 2**-8, 1.0 + 2**-7, 1.0 + 3 * 2**-8, -2.5, 3.0e-39], np.float32)
+    got = M.bf16(x)
+    # 1 + 2^-8 is a tie -> even (1.0); 1 + 3*2^-8 is a tie -> even (1 + 2^-6)
+    np.testing.assert_array_equal(got[:4], [1.0, 1.0, 1.0 + 2**-7, 1.0 + 2**-6])
+    assert got[4] == -2.5
+
+
+def test_gradient_matches_finite_differences():
+    rng = np.random.default_rng(0)
+    d = 20
+    L = M.layout(d)
+    p = np.zeros(L["total"], np.float64)
+    for name, scale in (("w1", 0.3), ("b1", 0.1), ("w2", 0.1), ("b2", 0.1), ("w3", 0.2), ("b3", 0.1)):
+        o, s = L[name]
+        p[o:o + s] = rng.standard_normal(s) * scale
+    # padding entries (rows of W1 past d, columns of W3/b3 past 12) are zero
+    o, s = L["w1"]
+    w1 = p[o:o + s].reshape(L["kinp"], M.H)
+    w1[d:] = 0
+    o, s = L["w3"]
+    p[o:o + s].reshape(M.H, M.OUTP)[:, M.CODE:] = 0
+    o, s = L["b3"]
+    p[o + M.CODE:o + s] = 0
+    X = rng.standard_normal((50, d)).astype(np.float32)
+    B = 8
+    rows = rng.integers(0, 50, 3 * B)
+    loss, g = M.loss_grad(p, X, rows, B, d, use_bf16=False)
+    idx = []
+    for name in ("w1", "b1", "w2", "b2", "w3", "b3"):
+        o, s = L[name]
+        idx += list(rng.choice(np.arange(o, o + s), 6, replace=False))
+    eps = 1e-6
+    for i in idx:
+        if g[i] == 0.0 and abs(p[i]) == 0.0:
+            continue
+        pp, pm = p.copy(), p.copy()
+        pp[i] += eps
+        pm[i] -= eps
+        fd = (M.loss_grad(pp, X, rows, B, d, False)[0] - M.loss_grad(pm, X, rows, B, d, False)[0]) / (2 * eps)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (i, fd, g[i])
