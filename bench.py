@@ -40,6 +40,9 @@ def parse():
     ap.add_argument("--queries", type=int, default=10000)
     ap.add_argument("--cpu-sample", type=int, default=100000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="time the plain (random-init) form only")
+    ap.add_argument("--learned-subsample", type=int, default=200000)
+    ap.add_argument("--target", type=float, default=0.95, help="recall@32 a timed form must reach")
     return ap.parse_args()
 
 
@@ -68,52 +71,117 @@ def cpu_baseline(X, sample_n):
                       f"counts + anti-hubs + core + MRD CSR + Kruskal + edge sort, C oracle with OpenMP"}
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print(f"bench.py runs the single-GPU path only (--gpus {a.gpus} requested); the multi-GPU sharding is a "
-              "design note, not built", file=sys.stderr)
-        sys.exit(2)
-    from gpu_hdbscan_amd import api, datasets
+def recall_at_32(ctx, Xd, lists, q, gt_ids):
+    """recall@32 of device lists on query ids q against ground-truth ids."""
+    from gpu_hdbscan_amd import api
+    rows = api.gather_rows(ctx, lists, q)
+    lab = rows["label"]
+    hit = sum(len(np.intersect1d(lab[i], gt_ids[i])) for i in range(len(q)))
+    return hit / (len(q) * 32.0)
 
-    X = datasets.make(a.workload)
-    n, d = X.shape
-    ctx = api.Context(0)
-    Xd = ctx.to_device(X)
-    p = api.default_params(nnd_iteration=a.iterations)
-    bufs = {"lists": ctx.empty((n, 32), api.NND_ELEMENT), "counts": ctx.empty((n,), np.int32),
-            "core": ctx.empty((n,), np.float32), "edges": ctx.empty((n - 1,), api.MST_EDGE)}
-    mode = 1 if a.blocked else 0
 
-    def step():
-        return api.pipeline(ctx, Xd, params=p, knng_mode=mode, blocks=max(a.blocked, 1), keep_on_device=True,
-                            outputs=bufs)
-
-    for _ in range(a.warmup):
+def time_leg(ctx, step, steps, warmup):
+    """W untimed steps, then K timed steps on the library stream; returns
+    (ms per step, the last result, HIP-event timers, NN-Descent stats)."""
+    for _ in range(warmup):
         step()
     ctx.synchronize()
     ctx.timer_enable(True)
     ctx.timer_reset()
     t0 = time.perf_counter()
-    for _ in range(a.steps):
+    res = None
+    for _ in range(steps):
         res = step()
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
     timers = ctx.timer_read()
     stats = ctx.nnd_stats()
     ctx.timer_enable(False)
-    ms_per_step = elapsed * 1e3 / a.steps
+    return elapsed * 1e3 / steps, res, timers, stats
 
-    # recall@32 of the last step against an exact FP32 brute force (GPU)
+
+def min_iterations(run, check, hi, target):
+    """Smallest NN-Descent iteration count in 1..hi whose lists pass the
+    recall target (runs once per count); hi when none does."""
+    for t in range(1, hi + 1):
+        run(t)
+        r = check()
+        if r >= target:
+            return t, r
+    return hi, r
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs the single-GPU path only (--gpus {a.gpus} requested); the multi-GPU sharding is a "
+              "design note, not built", file=sys.stderr)
+        sys.exit(2)
+    from gpu_hdbscan_amd import api, datasets, learned
+
+    X = datasets.make(a.workload)
+    n, d = X.shape
+    ctx = api.Context(0)
+    Xd = ctx.to_device(X)
+    bufs = {"lists": ctx.empty((n, 32), api.NND_ELEMENT), "counts": ctx.empty((n,), np.int32),
+            "core": ctx.empty((n,), np.float32), "edges": ctx.empty((n - 1,), api.MST_EDGE)}
+    mode = 1 if a.blocked else 0
     rng = np.random.default_rng(1234)
     q = rng.choice(n, size=min(a.queries, n), replace=False).astype(np.int32)
     gt_ids, _ = api.bruteforce_knn(ctx, Xd, q, 32)
-    lists = bufs["lists"].numpy()
-    lab = lists["label"][q]
-    hit = sum(len(np.intersect1d(lab[i], gt_ids[i], assume_unique=False)) for i in range(len(q)))
-    recall = hit / (len(q) * 32.0)
 
-    # dominant kernel: the local join, timed by HIP events on the library stream
+    def check():
+        return recall_at_32(ctx, Xd, bufs["lists"], q, gt_ids)
+
+    # plain form: random init, the reference's NND_ITERATION
+    p_plain = api.default_params(nnd_iteration=a.iterations)
+
+    def plain_step(prm=p_plain):
+        return api.pipeline(ctx, Xd, params=prm, knng_mode=mode, blocks=max(a.blocked, 1), keep_on_device=True,
+                            outputs=bufs)
+
+    ms_plain, res_plain, timers, stats = time_leg(ctx, plain_step, a.steps, a.warmup)
+    recall_plain = check()
+
+    legs = {"plain": {"iterations": a.iterations, "ms_per_step": round(ms_plain, 3),
+                      "points_per_s": n / (ms_plain * 1e-3), "recall_at_32": round(recall_plain, 4)}}
+    value, ms_step, recall, init_used = n / (ms_plain * 1e-3), ms_plain, recall_plain, "random"
+
+    if not a.no_learned and not a.blocked:
+        # plain form at the fewest iterations that still meet the recall target
+        t_pm, _ = min_iterations(lambda t: plain_step(api.default_params(nnd_iteration=t)), check, a.iterations,
+                                 a.target)
+        p_pm = api.default_params(nnd_iteration=t_pm)
+        ms_pm, _, _, _ = time_leg(ctx, lambda: plain_step(p_pm), a.steps, 1)
+        r_pm = check()
+        legs["plain_min_iterations"] = {"iterations": t_pm, "ms_per_step": round(ms_pm, 3),
+                                        "points_per_s": n / (ms_pm * 1e-3), "recall_at_32": round(r_pm, 4)}
+        # learned form: MLP trained on the GPU from plain NN-Descent over a
+        # seeded subsample, then learned initial lists + fewer iterations
+        model = learned.LearnedInit(ctx, d, seed=0).fit(X, n_sub=min(a.learned_subsample, n))
+        ctx.synchronize()
+
+        def learned_step(prm):
+            return learned.pipeline_learned(ctx, Xd, model, params=prm, outputs=bufs)
+
+        t_l, _ = min_iterations(lambda t: learned_step(api.default_params(nnd_iteration=t)), check, a.iterations,
+                                a.target)
+        p_l = api.default_params(nnd_iteration=t_l)
+        ms_l, res_l, timers_l, stats_l = time_leg(ctx, lambda: learned_step(p_l), a.steps, 1)
+        r_l = check()
+        legs["learned"] = {"iterations": t_l, "ms_per_step": round(ms_l, 3), "points_per_s": n / (ms_l * 1e-3),
+                           "recall_at_32": round(r_l, 4), "train_s": round(model.train_seconds, 3),
+                           "train_subsample": min(a.learned_subsample, n),
+                           "final_loss": round(float(model.final_loss), 5),
+                           "points_per_s_incl_training": n / (ms_l * 1e-3 + model.train_seconds)}
+        legs["learned_vs_plain"] = round(ms_plain / ms_l, 3)
+        legs["learned_vs_plain_min_iterations"] = round(ms_pm / ms_l, 3)
+        if r_l >= a.target:
+            value, ms_step, recall, init_used = n / (ms_l * 1e-3), ms_l, r_l, "learned"
+        model.close()
+
+    # dominant kernel of the plain leg: the local join, timed by HIP events
+    # on the library stream
     join_ms, join_launches = timers.get("nnd_join", (0.0, 0))
     iters = a.iterations
     per_launch_ms = join_ms / max(join_launches, 1)
@@ -135,28 +203,30 @@ def main():
     kern_ms = {k: round(v[0] / a.steps, 3) for k, v in sorted(timers.items()) if not k.startswith("stage_")}
     out = {
         "metric": "kNNG->MST points/sec at recall@32>=0.95, learned-init vs plain, 1 GPU",
-        "value": n / (ms_per_step * 1e-3),
+        "value": value,
         "unit": "points/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": round(ms_step, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": a.workload, "points": n, "dim": d, "k": 32, "sample_num": 16,
-                   "nnd_iteration": iters, "mpts": 32, "init": "random",
+                   "nnd_iteration": iters, "mpts": 32, "init": init_used,
                    "knng": f"blocked x{a.blocked}" if a.blocked else "single NN-Descent",
-                   "join_keep": int(p.join_keep)},
+                   "join_keep": int(p_plain.join_keep)},
         "recall_at_32": round(recall, 4),
+        "recall_ok": bool(recall >= a.target),
         "recall_queries": int(len(q)),
-        "mst_edges": int(res.n_edges),
-        "components": int(res.components),
-        "csr_edges": int(res.csr_edges),
-        "stage_ms": stage_ms,
-        "kernel_ms": kern_ms,
+        "legs": legs,
+        "mst_edges": int(res_plain.n_edges),
+        "components": int(res_plain.components),
+        "csr_edges": int(res_plain.csr_edges),
+        "stage_ms_plain": stage_ms,
+        "kernel_ms_plain": kern_ms,
         "roofline": {"kernel": "k_nnd_join", "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": traffic, "algorithmic_bytes_per_launch": int(join_bytes),

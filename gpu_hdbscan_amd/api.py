@@ -201,6 +201,14 @@ def GenLargeKNNGraph(ctx: Context, vectors, num_shards: int, params: HdbParams |
     return _ret(out, store_result_in_device)
 
 
+def gather_rows(ctx: Context, arr: "DeviceArray | np.ndarray", rows) -> np.ndarray:
+    """Host copy of the given rows of a (device or host) 2-D array."""
+    rows = np.asarray(rows, np.int64)
+    if isinstance(arr, DeviceArray):
+        arr = arr.numpy()
+    return np.ascontiguousarray(arr[rows])
+
+
 def bruteforce_knn(ctx: Context, vectors, query_ids, k: int = 32):
     """Exact FP32 kNN of the given query ids (sampled recall check)."""
     X, n, d = _as_points(ctx, vectors)

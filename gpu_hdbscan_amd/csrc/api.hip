@@ -344,7 +344,9 @@ int hdb_pipeline(hdb_ctx* ctx, const float* X, int64_t n, int32_t d, const hdb_p
   {
     Stage st(ctx, "stage_knng");
     if (o.knng_mode == 1) HDB_CALL(nnd_blocked(ctx, X, n, d, o.blocks, *params, lists));
-    else HDB_CALL(nnd_run(ctx, X, n, d, *params, lists, true, params->nnd_iteration));
+    else if (o.knng_mode == 2) HDB_CALL(nnd_run(ctx, X, n, d, *params, lists, false, params->nnd_iteration));
+    else if (o.knng_mode == 0) HDB_CALL(nnd_run(ctx, X, n, d, *params, lists, true, params->nnd_iteration));
+    else return ctx->fail(HDB_EINVAL, "knng_mode must be 0, 1 or 2");
   }
   // buildEnhancedKNNG (graph.cu:108-176)
   const int64_t hmax = static_cast<int64_t>(std::sqrt(static_cast<double>(n))) + 2;

@@ -96,10 +96,10 @@ class LearnedInit:
         self.train_seconds = time.perf_counter() - t0
         return self
 
-    def init_lists(self, X, seed: int = 0) -> api.DeviceArray:
+    def init_lists(self, X, seed: int = 0, out: api.DeviceArray | None = None) -> api.DeviceArray:
         Xd = self.ctx.dev(X, np.float32)
         n = Xd.shape[0]
-        L = self.ctx.empty((n, 32), NND_ELEMENT)
+        L = out if out is not None else self.ctx.empty((n, 32), NND_ELEMENT)
         self.ctx._check(self.ctx.lib.hdb_learned_init(self.ctx.handle, self.handle, Xd.ptr, n, self.d,
                                                       C.c_uint64(seed), L.ptr))
         return L
@@ -122,6 +122,19 @@ def sampled_recall(ctx: api.Context, Xd: api.DeviceArray, lists: api.DeviceArray
     lab = lists.numpy()["label"][q]
     hit = sum(len(np.intersect1d(lab[i], gt[i])) for i in range(len(q)))
     return hit / (len(q) * 32.0)
+
+
+def pipeline_learned(ctx: api.Context, X, model: LearnedInit, params=None, outputs: dict | None = None,
+                     keep_on_device: bool = True) -> api.PipelineResult:
+    """The whole path with learned initial lists: hdb_learned_init, then
+    hdb_pipeline in knng_mode 2 (params.nnd_iteration NN-Descent iterations
+    from those lists), on the context stream."""
+    Xd = ctx.dev(X, np.float32)
+    bufs = dict(outputs or {})
+    if bufs.get("lists") is None:
+        bufs["lists"] = ctx.empty((Xd.shape[0], 32), NND_ELEMENT)
+    model.init_lists(Xd, out=bufs["lists"])
+    return api.pipeline(ctx, Xd, params=params, knng_mode=2, keep_on_device=keep_on_device, outputs=bufs)
 
 
 def nndescent_learned(ctx: api.Context, X, model: LearnedInit, iterations: int, target: float = 0.95,

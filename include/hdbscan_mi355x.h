@@ -187,7 +187,10 @@ int hdb_sort_edges(hdb_ctx* ctx, hdb_mst_edge* edges_dev, int64_t m);
 
 /* ---- whole path ------------------------------------------------------------- */
 typedef struct {
-  int32_t knng_mode;      /* 0: one NN-Descent over all points; 1: blocked all-pairs */
+  int32_t knng_mode;      /* 0: one NN-Descent over all points (random init);
+                             1: blocked all-pairs;
+                             2: NN-Descent from the lists already in lists_dev
+                                (e.g. written by hdb_learned_init) */
   int32_t blocks;         /* knng_mode 1: number of blocks */
   int32_t keep_csr;       /* nonzero: leave the MRD CSR in the context (testing) */
   int32_t reserved[5];
