@@ -303,20 +303,7 @@ int hdb_mst(hdb_ctx* ctx, const int64_t* nindex, const int32_t* nlist, const flo
   if (!nindex || !nlist || !w || !out || !m_out || !components_out || n < 1)
     return ctx->fail(HDB_EINVAL, "bad hdb_mst arguments");
   HDB_CALL(set_device(ctx));
-  int64_t E = 0;
-  HDB_CALL(read_i64(ctx, nindex + n, &E));
-  int64_t m = 0;
-  HDB_CALL(csr_upper(ctx, nindex, nlist, w, n, E, nullptr, nullptr, nullptr, &m));
-  int32_t *usrc, *udst;
-  float* uw;
-  uint8_t* flags;
-  HDB_CALL(scratch_t(ctx, kSlotPipeSrc, std::max<int64_t>(m, 1), &usrc));
-  HDB_CALL(scratch_t(ctx, kSlotPipeNlist, std::max<int64_t>(m, 1), &udst));
-  HDB_CALL(scratch_t(ctx, kSlotPipeW, std::max<int64_t>(m, 1), &uw));
-  HDB_CALL(scratch_t(ctx, kSlotMstFlag, std::max<int64_t>(m, 1), &flags));
-  HDB_CALL(csr_upper(ctx, nindex, nlist, w, n, E, usrc, udst, uw, &m));
-  HDB_CALL(mst_edges_boruvka(ctx, usrc, udst, uw, n, m, flags, components_out));
-  HDB_CALL(mst_extract(ctx, usrc, udst, uw, flags, m, out, std::max<int64_t>(n - 1, 0), m_out));
+  HDB_CALL(mst_csr(ctx, nindex, nlist, w, n, out, m_out, components_out));
   return *components_out > 1 ? ctx->fail(HDB_EDISCONNECTED, "graph is disconnected: spanning forest returned")
                              : HDB_OK;
 }
@@ -362,28 +349,25 @@ int hdb_pipeline(hdb_ctx* ctx, const float* X, int64_t n, int32_t d, const hdb_p
     Stage st(ctx, "stage_core");
     HDB_CALL(core_distances(ctx, X, lists, n, d, params->k, params->mpts, core));
   }
-  const int64_t cap = n * params->k + h * (h - 1) / 2 + 1;
+  // full (symmetric) CSR: every kNN edge on both rows plus the clique
+  const int64_t cap = 2 * n * params->k + h * (h - 1) + 1;
   int64_t* nindex;
-  int32_t *nlist, *src;
+  int32_t* nlist;
   float* w;
   HDB_CALL(scratch_t(ctx, kSlotPipeNindex, n + 1, &nindex));
   HDB_CALL(scratch_t(ctx, kSlotPipeNlist, cap, &nlist));
-  HDB_CALL(scratch_t(ctx, kSlotPipeSrc, cap, &src));
   HDB_CALL(scratch_t(ctx, kSlotPipeW, cap, &w));
-  int64_t m = 0;
+  int64_t E = 0;
   {
     Stage st(ctx, "stage_mrd_graph");
-    HDB_CALL(mrd_graph(ctx, X, lists, core, ah, h, n, d, params->k, true, nindex, nlist, w, src, &m));
+    HDB_CALL(mrd_graph(ctx, X, lists, core, ah, h, n, d, params->k, false, nindex, nlist, w, nullptr, &E));
   }
-  ctx->csr_edges = 2 * m;
-  if (csr_edges_out) *csr_edges_out = 2 * m;
+  ctx->csr_edges = E;
+  if (csr_edges_out) *csr_edges_out = E;
   // cpuMST/gpuMST + buildMST (_new.cu:298-314)
-  uint8_t* flags;
-  HDB_CALL(scratch_t(ctx, kSlotMstFlag, std::max<int64_t>(m, 1), &flags));
   {
     Stage st(ctx, "stage_mst");
-    HDB_CALL(mst_edges_boruvka(ctx, src, nlist, w, n, m, flags, components_out));
-    HDB_CALL(mst_extract(ctx, src, nlist, w, flags, m, edges, std::max<int64_t>(n - 1, 0), m_out));
+    HDB_CALL(mst_csr(ctx, nindex, nlist, w, n, edges, m_out, components_out));
   }
   {
     Stage st(ctx, "stage_sort");

@@ -28,9 +28,9 @@ enum Slot : int {
   kSlotScanTmp, kSlotRowCnt, kSlotRowCursor, kSlotRowTmp, kSlotRowUniq,  // CSR build
   kSlotAhFlag, kSlotAhAux, kSlotAhKeys, kSlotAhKeys2,                 // anti-hubs
   kSlotMstParent, kSlotMstMin, kSlotMstWl1, kSlotMstWl2, kSlotMstFlag, kSlotMstCnt,
-  kSlotMstSrc, kSlotMstPos,
+  kSlotMstSrc, kSlotMstPos, kSlotMstCw, kSlotSeedX,
   kSlotSortKeys, kSlotSortVals, kSlotSortHist, kSlotSortAux,          // radix sort
-  kSlotPipeNindex, kSlotPipeNlist, kSlotPipeW, kSlotPipeAh, kSlotPipeSrc,  // pipeline
+  kSlotPipeNindex, kSlotPipeNlist, kSlotPipeW, kSlotPipeAh,  // pipeline
   kSlotBfTmp, kSlotMisc0, kSlotMisc1, kSlotMisc2,
   kSlotPropTgt, kSlotPropSE, kSlotRevSE, kSlotPart,                     // sort-routed NN-Descent
   kSlotCount
@@ -295,16 +295,11 @@ int mrd_graph(hdb_ctx* ctx, const float* X, const hdb_nnd_element* lists, const 
               int64_t* nindex, int32_t* nlist, float* w, int32_t* src, int64_t* edges_out);
 
 // mst.hip
-// Minimum spanning forest of the undirected edge list (src[e] < dst[e], e in
-// canonical order) under the order (w, e).  flags[e] set for tree edges.
-int mst_edges_boruvka(hdb_ctx* ctx, const int32_t* src, const int32_t* dst, const float* w,
-                      int64_t n, int64_t m_edges, uint8_t* flags, int64_t* components_out);
-int mst_extract(hdb_ctx* ctx, const int32_t* src, const int32_t* dst, const float* w,
-                const uint8_t* flags, int64_t m_edges, hdb_mst_edge* out, int64_t cap, int64_t* count_out);
+// Minimum spanning forest of a full (symmetric) CSR under the order
+// (weight, min id, max id); out (capacity n-1) receives the edges in CSR
+// order (from < to).  Weights must not be NaN.
+int mst_csr(hdb_ctx* ctx, const int64_t* nindex, const int32_t* nlist, const float* w, int64_t n,
+            hdb_mst_edge* out, int64_t* m_out, int64_t* components_out);
 int sort_edges(hdb_ctx* ctx, hdb_mst_edge* edges, int64_t m);
-// Upper-triangle source/position arrays of a full CSR (entry j of row i with
-// nlist[j] > i): writes src, dst, w of those entries in CSR order.
-int csr_upper(hdb_ctx* ctx, const int64_t* nindex, const int32_t* nlist, const float* w, int64_t n,
-              int64_t E, int32_t* usrc, int32_t* udst, float* uw, int64_t* m_out);
 
 }  // namespace hdb

@@ -243,13 +243,16 @@ __global__ void k_mlp_reduce(const float* __restrict__ part, int64_t splits, int
   grad[i] = s;
 }
 
-// Bias gradients: column sums of a dense R x n matrix (one thread per column).
-__global__ void k_mlp_colsum(const float* __restrict__ G, int64_t R, int n, float* __restrict__ out) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+// Bias gradients, first pass: column sums of rows [z*chunk, (z+1)*chunk) of
+// a dense R x n matrix into part[z][c] (block z, one thread per column).
+__global__ void k_mlp_colsum(const float* __restrict__ G, int64_t R, int n, int64_t chunk, float* __restrict__ part) {
+  const int c = threadIdx.x;
   if (c >= n) return;
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * chunk;
+  const int64_t r1 = r0 + chunk < R ? r0 + chunk : R;
   float s = 0.0f;
-  for (int64_t r = 0; r < R; ++r) s += G[r * n + c];
-  out[c] = s;
+  for (int64_t r = r0; r < r1; ++r) s += G[r * n + c];
+  part[static_cast<int64_t>(blockIdx.x) * n + c] = s;
 }
 
 // Mean of tanh(z) per code bit over the R rows (one block).
@@ -355,23 +358,69 @@ __global__ void k_mlp_keys(const float* __restrict__ Z, const float* __restrict_
   ids[i] = static_cast<uint32_t>(i);
 }
 
-// Initial lists from the bucket order: the point at sorted position r takes
-// the points at positions r-16 .. r+16 (shifted inside [0, n)), self excluded.
-__global__ void k_mlp_windows(const uint32_t* __restrict__ order, int64_t n, hdb_nnd_element* __restrict__ lists) {
+// Points permuted into the bucket order: row r of Xs is point order[r].
+__global__ void k_mlp_permute(const float* __restrict__ X, const uint32_t* __restrict__ order, int64_t n, int d,
+                              float* __restrict__ Xs) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int64_t r = t >> 5;
-  if (r >= n) return;
-  const int j = static_cast<int>(t & 31);
-  int64_t lo = r - 16;
-  if (lo < 0) lo = 0;
-  if (lo + 32 >= n) lo = n - 33 < 0 ? 0 : n - 33;
-  int64_t pos = lo + j;
-  if (pos >= r) pos += 1;
-  const uint32_t self = order[r];
-  hdb_nnd_element e;
-  e.distance = 0.0f;
-  e.label = pos < n ? static_cast<int32_t>(order[pos]) : static_cast<int32_t>(kInvalidId);
-  lists[static_cast<int64_t>(self) * kK + j] = e;
+  if (t >= n * d) return;
+  const int64_t r = t / d;
+  const int k = static_cast<int>(t - r * d);
+  Xs[t] = X[static_cast<int64_t>(order[r]) * d + k];
+}
+
+constexpr int kSeedWin = 64;   // bucket-order neighbours examined per point
+constexpr int kSeedKeep = 24;  // nearest of them kept; the other 8 entries are random ids
+constexpr int kSeedPts = 32;   // points per block (4 waves x 8)
+constexpr int kSeedRows = kSeedPts + kSeedWin;
+
+__device__ __forceinline__ int64_t seed_lo(int64_t r, int64_t n) {
+  int64_t lo = r - kSeedWin / 2;
+  if (lo > n - kSeedWin - 1) lo = n - kSeedWin - 1;
+  return lo < 0 ? 0 : lo;
+}
+
+// Initial lists from the bucket order: the point at sorted position r takes
+// the kSeedKeep nearest of the kSeedWin points around it in that order
+// (exact squared distances, rows staged in LDS), plus 32 - kSeedKeep random
+// ids that keep the graph connected across buckets.
+__global__ __launch_bounds__(256) void k_mlp_seed(const float* __restrict__ Xs, const uint32_t* __restrict__ order,
+                                                  int64_t n, int d, uint64_t seed,
+                                                  hdb_nnd_element* __restrict__ lists) {
+  extern __shared__ float rows[];  // kSeedRows x (d + 1)
+  const int dp = d + 1;
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kSeedPts;
+  const int64_t sb = seed_lo(r0, n);
+  int64_t se = seed_lo(r0 + kSeedPts - 1, n) + kSeedWin + 1;
+  if (se > n) se = n;
+  for (int64_t t = threadIdx.x; t < (se - sb) * d; t += blockDim.x) {
+    const int64_t rr = t / d;
+    rows[rr * dp + (t - rr * d)] = Xs[sb * d + t];
+  }
+  __syncthreads();
+  const unsigned lane = __lane_id();
+  const int wv = static_cast<int>(threadIdx.x >> 6);
+  for (int q = 0; q < kSeedPts / 4; ++q) {
+    const int64_t r = r0 + wv * (kSeedPts / 4) + q;
+    if (r >= n) break;  // uniform across the wave
+    const int64_t lo = seed_lo(r, n);
+    int64_t p = lo + lane;
+    if (p >= r) p += 1;
+    uint64_t key = kKeyMax;
+    if (p < n) key = make_key(d2_any(rows + (p - sb) * dp, rows + (r - sb) * dp, d), order[p], 1u);
+    key = bitonic_group<64>(key);
+    const uint32_t self = order[r];
+    hdb_nnd_element e;
+    if (lane < static_cast<unsigned>(kSeedKeep)) {
+      e = key == kKeyMax ? hdb_nnd_element{__uint_as_float(0x7F800000u), static_cast<int32_t>(kInvalidId)}
+                         : hdb_nnd_element{key_d2(key), static_cast<int32_t>(key_id(key))};
+    } else {
+      uint64_t x = xorshift64star(seed + static_cast<uint64_t>(self) * kK + lane + 1);
+      uint32_t id = static_cast<uint32_t>(x % static_cast<uint64_t>(n));
+      if (id == self) id = static_cast<uint32_t>((id + 1) % static_cast<uint64_t>(n));
+      e = hdb_nnd_element{0.0f, static_cast<int32_t>(id)};
+    }
+    if (lane < static_cast<unsigned>(kK)) lists[static_cast<int64_t>(self) * kK + lane] = e;
+  }
 }
 
 }  // namespace hdb
@@ -431,6 +480,20 @@ int wgrad(hdb_ctx* ctx, const float* A, int lda, const float* X, int d, const in
   return HDB_OK;
 }
 
+// Column sums of an R x n matrix (n <= 128) into out, in a fixed order:
+// 256-row partial sums, then their sum.  part holds ceil(R/256) * n floats.
+int colsum(hdb_ctx* ctx, const float* G, int64_t R, int n, float* part, float* out) {
+  const int64_t chunk = 256;
+  const int64_t splits = (R + chunk - 1) / chunk;
+  hipLaunchKernelGGL(k_mlp_colsum, dim3(static_cast<unsigned>(splits)), dim3(kHid), 0, ctx->stream, G, R, n, chunk,
+                     part);
+  HDB_LAUNCHED();
+  hipLaunchKernelGGL(k_mlp_reduce, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, part, splits,
+                     static_cast<int64_t>(n), out);
+  HDB_LAUNCHED();
+  return HDB_OK;
+}
+
 // One loss + gradient evaluation for the triples in `rows` (3B rows).
 int mlp_loss_grad(hdb_ctx* ctx, hdb_mlp* net, const float* X, int d, const int32_t* rows, int64_t B, double* loss) {
   const MlpLayout& L = net->L;
@@ -461,12 +524,9 @@ int mlp_loss_grad(hdb_ctx* ctx, hdb_mlp* net, const float* X, int d, const int32
   HDB_CALL(wgrad(ctx, H2, kHid, nullptr, 0, nullptr, dZ, kOutP, R, kHid, kOutP, part, g + L.w3));
   HDB_CALL(wgrad(ctx, H1, kHid, nullptr, 0, nullptr, dP2, kHid, R, kHid, kHid, part, g + L.w2));
   HDB_CALL(wgrad(ctx, nullptr, 0, X, d, rows, dP1, kHid, R, L.kinp, kHid, part, g + L.w1));
-  hipLaunchKernelGGL(k_mlp_colsum, dim3(1), dim3(kOutP), 0, ctx->stream, dZ, R, kOutP, g + L.b3);
-  HDB_LAUNCHED();
-  hipLaunchKernelGGL(k_mlp_colsum, dim3(1), dim3(kHid), 0, ctx->stream, dP2, R, kHid, g + L.b2);
-  HDB_LAUNCHED();
-  hipLaunchKernelGGL(k_mlp_colsum, dim3(1), dim3(kHid), 0, ctx->stream, dP1, R, kHid, g + L.b1);
-  HDB_LAUNCHED();
+  HDB_CALL(colsum(ctx, dZ, R, kOutP, part, g + L.b3));
+  HDB_CALL(colsum(ctx, dP2, R, kHid, part, g + L.b2));
+  HDB_CALL(colsum(ctx, dP1, R, kHid, part, g + L.b1));
   if (loss) {
     std::vector<float> lp(B);
     float mh[kCode];
@@ -630,7 +690,13 @@ int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* net, const float* X, int64_t n, int3
   hipLaunchKernelGGL(k_mlp_keys, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, Z, X, d, u, 64.0f, n, keys, ids);
   HDB_LAUNCHED();
   HDB_CALL(radix_sort_u32_u32(ctx, keys, ids, n, 0, 32));
-  hipLaunchKernelGGL(k_mlp_windows, dim3(grid_for(n * kK, 256)), dim3(256), 0, ctx->stream, ids, n, lists);
+  float* Xs;
+  HDB_CALL(scratch_t(ctx, kSlotSeedX, n * d, &Xs));
+  hipLaunchKernelGGL(k_mlp_permute, dim3(grid_for(n * d, 256)), dim3(256), 0, ctx->stream, X, ids, n, d, Xs);
+  HDB_LAUNCHED();
+  const size_t lds = static_cast<size_t>(kSeedRows) * (d + 1) * sizeof(float);
+  hipLaunchKernelGGL(k_mlp_seed, dim3(grid_for(n, kSeedPts)), dim3(256), lds, ctx->stream, Xs, ids, n, d, seed,
+                     lists);
   HDB_LAUNCHED();
   return HDB_OK;
 }

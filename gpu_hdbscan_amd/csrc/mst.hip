@@ -1,17 +1,31 @@
 // Boruvka minimum spanning forest, MST-edge extraction and the stable
 // MST-edge sort for gfx950.
 //
-// Edges are undirected, given once as (src < dst, w) in a canonical order e.
-// The strict total order is the 64-bit key (float bits of w) << 32 | e: w >= 0
-// so the bit pattern orders like the value, and e breaks ties the way the
-// reference's Kruskal sorts (w, CSR position) (HDBSCAN/mst/mst.cu:42-54).
-// Weights are never quantised (the reference's GPU path scales to int,
-// HDBSCAN/core-sg/core-sg.cu:13-35).
+// The forest is computed on the full (symmetric) CSR, vertex by vertex: in
+// each round every live vertex scans its own row (sixteen lanes per vertex)
+// for the lightest edge leaving its component under the strict order
+// (weight, min id, max id), which is the order of the reference's Kruskal,
+// (w, CSR position of the from < to entry) over rows sorted by id
+// (HDBSCAN/mst/mst.cu:34-78).  Components then agree on their lightest edge
+// with two per-vertex atomicMin passes (weight bits, then the id pair), so
+// atomics scale with vertices, not edges.  Weights are never quantised (the
+// reference's GPU path scales them to int, HDBSCAN/core-sg/core-sg.cu:13-35).
 #include <algorithm>
 
 #include "hdb_internal.hpp"
 
 namespace hdb {
+
+constexpr uint32_t kNoEdge = 0xFFFFFFFFu;  // above every mapped weight
+
+// Order-preserving map of float bits to u32 (any sign; NaN excluded).
+__device__ __forceinline__ uint32_t wkey(float w) {
+  const uint32_t b = __float_as_uint(w);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float wval(uint32_t k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
 
 __device__ __forceinline__ int find_compress(int v, int* __restrict__ parent) {
   while (true) {
@@ -24,59 +38,7 @@ __device__ __forceinline__ int find_compress(int v, int* __restrict__ parent) {
   }
 }
 
-__device__ __forceinline__ int find_volatile(int v, volatile int* parent) {
-  int p = parent[v];
-  while (p != v) {
-    v = p;
-    p = parent[v];
-  }
-  return v;
-}
-
-// Hook the larger root under the smaller (join, mst.cu:93-100).
-__device__ __forceinline__ void link_roots(int a, int b, int* parent) {
-  volatile int* vp = parent;
-  while (true) {
-    a = find_volatile(a, vp);
-    b = find_volatile(b, vp);
-    if (a == b) return;
-    if (a < b) {
-      const int t = a;
-      a = b;
-      b = t;
-    }
-    if (atomicCAS(&parent[a], a, b) == a) return;
-  }
-}
-
-__device__ __forceinline__ uint64_t edge_key(uint32_t wbits, uint32_t e) {
-  return (static_cast<uint64_t>(wbits) << 32) | e;
-}
-
-// Round 0: every vertex is its own component.
-__global__ void k_mst_min0(const int32_t* __restrict__ src, const int32_t* __restrict__ dst,
-                           const float* __restrict__ w, int64_t m, unsigned long long* __restrict__ minkey) {
-  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= m) return;
-  const uint64_t key = edge_key(__float_as_uint(w[e]), static_cast<uint32_t>(e));
-  atomicMin(&minkey[src[e]], static_cast<unsigned long long>(key));
-  atomicMin(&minkey[dst[e]], static_cast<unsigned long long>(key));
-}
-
-__global__ void k_mst_hook0(const int32_t* __restrict__ src, const int32_t* __restrict__ dst,
-                            const float* __restrict__ w, int64_t m, const unsigned long long* __restrict__ minkey,
-                            int* __restrict__ parent, uint8_t* __restrict__ flags) {
-  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= m) return;
-  const int a = src[e], b = dst[e];
-  const uint64_t key = edge_key(__float_as_uint(w[e]), static_cast<uint32_t>(e));
-  if (key == minkey[a] || key == minkey[b]) {
-    flags[e] = 1;
-    link_roots(a, b, parent);
-  }
-}
-
-// Wave-aggregated append to a worklist.
+// Wave-aggregated append to a list.
 __device__ __forceinline__ int64_t wave_append(bool want, unsigned long long* counter) {
   const uint64_t bal = __ballot(want);
   if (!bal) return -1;
@@ -87,65 +49,107 @@ __device__ __forceinline__ int64_t wave_append(bool want, unsigned long long* co
   return want ? static_cast<int64_t>(base) + __popcll(bal & lanemask_lt()) : -1;
 }
 
-// Round 1 from the edge arrays: drop intra-component edges, compact the
-// rest into the worklist with their current roots, offer keys (kernel1,
-// mst.cu:155-171).
-__global__ void k_mst_min_arr(const int32_t* __restrict__ src, const int32_t* __restrict__ dst,
-                              const float* __restrict__ w, int64_t m, int* __restrict__ parent,
-                              unsigned long long* __restrict__ minkey, uint4* __restrict__ wl,
-                              unsigned long long* __restrict__ wl_cnt) {
-  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  bool keep = false;
-  int ra = 0, rb = 0;
-  uint32_t wb = 0;
-  if (e < m) {
-    ra = find_compress(src[e], parent);
-    rb = find_compress(dst[e], parent);
-    keep = ra != rb;
-    wb = __float_as_uint(w[e]);
+// Lightest edge leaving the component of each live vertex (sixteen lanes
+// per row).  A vertex whose row has no such edge stays dead: components only
+// grow.  Offers the weight to the component (cw), counts vertices with an
+// edge in `active`.
+__global__ __launch_bounds__(256) void k_bv_best(const int64_t* __restrict__ nindex, const int32_t* __restrict__ nlist,
+                                                 const float* __restrict__ w, int64_t n, const int32_t* __restrict__ comp,
+                                                 uint32_t* __restrict__ best_w, uint64_t* __restrict__ best_p,
+                                                 uint32_t* __restrict__ cw, unsigned long long* __restrict__ active) {
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * 16 + (threadIdx.x >> 4);
+  const int sl = static_cast<int>(threadIdx.x & 15);
+  uint32_t bw = kNoEdge;
+  uint64_t bp = ~0ull;
+  int rv = 0;
+  const bool live = v < n && best_w[v] != kNoEdge;
+  if (live) {
+    rv = comp[v];
+    const int64_t e1 = nindex[v + 1];
+    for (int64_t j = nindex[v] + sl; j < e1; j += 16) {
+      const int32_t u = nlist[j];
+      if (u < 0 || static_cast<int64_t>(u) >= n || comp[u] == rv) continue;
+      const uint32_t wb = wkey(w[j]);
+      const uint64_t pr = v < u ? ((static_cast<uint64_t>(v) << 32) | static_cast<uint32_t>(u))
+                                : ((static_cast<uint64_t>(u) << 32) | static_cast<uint64_t>(v));
+      if (wb < bw || (wb == bw && pr < bp)) {
+        bw = wb;
+        bp = pr;
+      }
+    }
   }
-  const int64_t slot = wave_append(keep, wl_cnt);
-  if (keep) {
-    wl[slot] = make_uint4(static_cast<uint32_t>(ra), static_cast<uint32_t>(rb), wb, static_cast<uint32_t>(e));
-    const uint64_t key = edge_key(wb, static_cast<uint32_t>(e));
-    atomicMin(&minkey[ra], static_cast<unsigned long long>(key));
-    atomicMin(&minkey[rb], static_cast<unsigned long long>(key));
+#pragma unroll
+  for (int o = 8; o > 0; o >>= 1) {
+    const uint32_t ow = __shfl_xor(bw, o);
+    const uint64_t op = shfl_xor_u64(bp, o);
+    if (ow < bw || (ow == bw && op < bp)) {
+      bw = ow;
+      bp = op;
+    }
+  }
+  const bool lead = live && sl == 0;
+  if (lead) {
+    best_w[v] = bw;
+    best_p[v] = bp;
+    if (bw != kNoEdge) atomicMin(&cw[rv], bw);
+  }
+  const uint64_t bal = __ballot(lead && bw != kNoEdge);
+  if (__lane_id() == 0 && bal) atomicAdd(active, static_cast<unsigned long long>(__popcll(bal)));
+}
+
+// Among the vertices that offered the component's weight, the smallest pair.
+__global__ void k_bv_pair(int64_t n, const int32_t* __restrict__ comp, const uint32_t* __restrict__ best_w,
+                          const uint64_t* __restrict__ best_p, const uint32_t* __restrict__ cw,
+                          unsigned long long* __restrict__ cp) {
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const uint32_t bw = best_w[v];
+  if (bw == kNoEdge) return;
+  const int rv = comp[v];
+  if (bw == cw[rv]) atomicMin(&cp[rv], static_cast<unsigned long long>(best_p[v]));
+}
+
+// The vertex holding its component's lightest edge records it and hooks its
+// root to the other side's root; when two components chose the same edge the
+// smaller root records it and the larger hooks under it (kernel2/kernel3,
+// mst.cu:174-196).  Roots are read from comp and written to next only.
+__global__ void k_bv_hook(int64_t n, const int32_t* __restrict__ comp, const uint32_t* __restrict__ best_w,
+                          const uint64_t* __restrict__ best_p, const uint32_t* __restrict__ cw,
+                          const unsigned long long* __restrict__ cp, int32_t* __restrict__ next,
+                          hdb_mst_edge* __restrict__ out, int64_t cap, unsigned long long* __restrict__ out_cnt) {
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  bool rec = false;
+  uint32_t bw = kNoEdge;
+  uint64_t pr = 0;
+  if (v < n) {
+    bw = best_w[v];
+    if (bw != kNoEdge) {
+      const int rv = comp[v];
+      pr = best_p[v];
+      if (bw == cw[rv] && pr == cp[rv]) {
+        const int a = static_cast<int>(pr >> 32), b = static_cast<int>(static_cast<uint32_t>(pr));
+        const int u = a == static_cast<int>(v) ? b : a;
+        const int ru = comp[u];
+        const bool mutual = cw[ru] == bw && cp[ru] == pr;
+        rec = !mutual || rv < ru;
+        if (!mutual || rv > ru) next[rv] = ru;
+      }
+    }
+  }
+  const int64_t slot = wave_append(rec, out_cnt);
+  if (rec && slot < cap) {
+    hdb_mst_edge e;
+    e.from_node = static_cast<int32_t>(pr >> 32);
+    e.to_node = static_cast<int32_t>(static_cast<uint32_t>(pr));
+    e.weight = wval(bw);
+    out[slot] = e;
   }
 }
 
-__global__ void k_mst_min_wl(const uint4* __restrict__ wl_in, int64_t cnt, int* __restrict__ parent,
-                             unsigned long long* __restrict__ minkey, uint4* __restrict__ wl_out,
-                             unsigned long long* __restrict__ wl_cnt) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  bool keep = false;
-  uint4 el = make_uint4(0, 0, 0, 0);
-  if (i < cnt) {
-    el = wl_in[i];
-    el.x = static_cast<uint32_t>(find_compress(static_cast<int>(el.x), parent));
-    el.y = static_cast<uint32_t>(find_compress(static_cast<int>(el.y), parent));
-    keep = el.x != el.y;
-  }
-  const int64_t slot = wave_append(keep, wl_cnt);
-  if (keep) {
-    wl_out[slot] = el;
-    const uint64_t key = edge_key(el.z, el.w);
-    atomicMin(&minkey[el.x], static_cast<unsigned long long>(key));
-    atomicMin(&minkey[el.y], static_cast<unsigned long long>(key));
-  }
-}
-
-// kernel2 (mst.cu:174-185): an edge that is the minimum of either endpoint
-// component joins the forest.
-__global__ void k_mst_hook_wl(const uint4* __restrict__ wl, int64_t cnt, const unsigned long long* __restrict__ minkey,
-                              int* __restrict__ parent, uint8_t* __restrict__ flags) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= cnt) return;
-  const uint4 el = wl[i];
-  const uint64_t key = edge_key(el.z, el.w);
-  if (key == minkey[el.x] || key == minkey[el.y]) {
-    flags[el.w] = 1;
-    link_roots(static_cast<int>(el.x), static_cast<int>(el.y), parent);
-  }
+// New root of every vertex (path halving on next).
+__global__ void k_bv_jump(int64_t n, int32_t* __restrict__ comp, int32_t* __restrict__ next) {
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v < n) comp[v] = find_compress(comp[v], next);
 }
 
 __global__ void k_iota(int* __restrict__ p, int64_t n) {
@@ -153,185 +157,113 @@ __global__ void k_iota(int* __restrict__ p, int64_t n) {
   if (i < n) p[i] = static_cast<int>(i);
 }
 
-__global__ void k_count_roots(const int* __restrict__ parent, int64_t n, unsigned long long* __restrict__ cnt) {
+__global__ void k_count_roots(const int* __restrict__ comp, int64_t n, unsigned long long* __restrict__ cnt) {
   const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const bool root = v < n && parent[v] == static_cast<int>(v);
+  const bool root = v < n && comp[v] == static_cast<int>(v);
   const uint64_t bal = __ballot(root);
   if (__lane_id() == 0 && bal) atomicAdd(cnt, static_cast<unsigned long long>(__popcll(bal)));
 }
 
-int mst_edges_boruvka(hdb_ctx* ctx, const int32_t* src, const int32_t* dst, const float* w, int64_t n,
-                      int64_t m, uint8_t* flags, int64_t* components_out) {
-  if (n < 1 || m < 0) return ctx->fail(HDB_EINVAL, "bad MST sizes");
-  if (m >= (1ll << 32)) return ctx->fail(HDB_EUNSUPPORTED, "more than 2^32 undirected edges");
+// CSR order of the from < to entries: stable passes by `to`, then `from`.
+__global__ void k_csr_split(const hdb_mst_edge* __restrict__ e, int64_t m, uint32_t* __restrict__ keys,
+                            uint64_t* __restrict__ vals) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const hdb_mst_edge x = e[i];
+  keys[i] = static_cast<uint32_t>(x.to_node);
+  vals[i] = (static_cast<uint64_t>(static_cast<uint32_t>(x.from_node)) << 32) | __float_as_uint(x.weight);
+}
+
+__global__ void k_csr_swap(uint32_t* __restrict__ keys, uint64_t* __restrict__ vals, int64_t m) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const uint32_t to = keys[i];
+  const uint64_t v = vals[i];
+  keys[i] = static_cast<uint32_t>(v >> 32);
+  vals[i] = (static_cast<uint64_t>(to) << 32) | static_cast<uint32_t>(v);
+}
+
+__global__ void k_csr_join(const uint32_t* __restrict__ keys, const uint64_t* __restrict__ vals, int64_t m,
+                           hdb_mst_edge* __restrict__ e) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  hdb_mst_edge x;
+  x.from_node = static_cast<int32_t>(keys[i]);
+  x.to_node = static_cast<int32_t>(vals[i] >> 32);
+  x.weight = __uint_as_float(static_cast<uint32_t>(vals[i]));
+  e[i] = x;
+}
+
+int mst_csr(hdb_ctx* ctx, const int64_t* nindex, const int32_t* nlist, const float* w, int64_t n,
+            hdb_mst_edge* out, int64_t* m_out, int64_t* components_out) {
+  if (n < 1) return ctx->fail(HDB_EINVAL, "bad MST sizes");
+  if (n >= (1ll << 31)) return ctx->fail(HDB_EUNSUPPORTED, "more than 2^31 vertices");
   Stage st(ctx, "mst_boruvka");
-  int* parent;
-  unsigned long long* minkey;
-  unsigned long long* cnt;
-  HDB_CALL(scratch_t(ctx, kSlotMstParent, n, &parent));
-  HDB_CALL(scratch_t(ctx, kSlotMstMin, n, &minkey));
+  int32_t *comp, *next;
+  uint32_t *best_w, *cw;
+  uint64_t* best_p;
+  unsigned long long *cp, *cnt;
+  HDB_CALL(scratch_t(ctx, kSlotMstParent, n, &comp));
+  HDB_CALL(scratch_t(ctx, kSlotMstFlag, n, &next));
+  HDB_CALL(scratch_t(ctx, kSlotMstWl2, n, &best_w));
+  HDB_CALL(scratch_t(ctx, kSlotMstCw, n, &cw));
+  HDB_CALL(scratch_t(ctx, kSlotMstWl1, n, &best_p));
+  HDB_CALL(scratch_t(ctx, kSlotMstMin, n, &cp));
   HDB_CALL(scratch_t(ctx, kSlotMstCnt, 4, &cnt));
-  hipLaunchKernelGGL(k_iota, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, parent, n);
+  const unsigned gv = grid_for(n, 256);
+  hipLaunchKernelGGL(k_iota, dim3(gv), dim3(256), 0, ctx->stream, comp, n);
   HDB_LAUNCHED();
-  HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(minkey), n, kKeyMax));
-  HDB_CALL(fill_u8(ctx, flags, m, 0));
-  if (m > 0) {
-    const unsigned g = grid_for(m, 256);
-    hipLaunchKernelGGL(k_mst_min0, dim3(g), dim3(256), 0, ctx->stream, src, dst, w, m, minkey);
+  hipLaunchKernelGGL(k_iota, dim3(gv), dim3(256), 0, ctx->stream, next, n);
+  HDB_LAUNCHED();
+  HDB_CALL(fill_i32(ctx, reinterpret_cast<int32_t*>(best_w), n, 0));  // every vertex starts live
+  HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(cnt), 2, 0));    // cnt[1]: recorded edges
+  const int64_t cap = n - 1;
+  for (int round = 0;; ++round) {
+    if (round > 64) return ctx->fail(HDB_EDEVICE, "Boruvka did not converge");
+    HDB_CALL(fill_i32(ctx, reinterpret_cast<int32_t*>(cw), n, static_cast<int32_t>(kNoEdge)));
+    HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(cp), n, ~0ull));
+    HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(cnt), 1, 0));
+    hipLaunchKernelGGL(k_bv_best, dim3(grid_for(n, 16)), dim3(256), 0, ctx->stream, nindex, nlist, w, n, comp, best_w,
+                       best_p, cw, cnt);
     HDB_LAUNCHED();
-    hipLaunchKernelGGL(k_mst_hook0, dim3(g), dim3(256), 0, ctx->stream, src, dst, w, m, minkey, parent, flags);
+    int64_t active = 0;
+    HDB_CALL(read_i64(ctx, reinterpret_cast<const int64_t*>(cnt), &active));
+    if (active == 0) break;
+    hipLaunchKernelGGL(k_bv_pair, dim3(gv), dim3(256), 0, ctx->stream, n, comp, best_w, best_p, cw, cp);
     HDB_LAUNCHED();
-    uint4 *wl_a, *wl_b;
-    HDB_CALL(scratch_t(ctx, kSlotMstWl1, m, &wl_a));
-    HDB_CALL(scratch_t(ctx, kSlotMstWl2, m, &wl_b));
-    int64_t count = -1;
-    for (int round = 1;; ++round) {
-      HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(minkey), n, kKeyMax));
-      HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(cnt), 1, 0));
-      if (round == 1) {
-        hipLaunchKernelGGL(k_mst_min_arr, dim3(g), dim3(256), 0, ctx->stream, src, dst, w, m, parent, minkey, wl_a,
-                           cnt);
-      } else {
-        hipLaunchKernelGGL(k_mst_min_wl, dim3(grid_for(count, 256)), dim3(256), 0, ctx->stream, wl_b, count, parent,
-                           minkey, wl_a, cnt);
-      }
-      HDB_LAUNCHED();
-      HDB_CALL(read_i64(ctx, reinterpret_cast<const int64_t*>(cnt), &count));
-      if (count == 0) break;
-      hipLaunchKernelGGL(k_mst_hook_wl, dim3(grid_for(count, 256)), dim3(256), 0, ctx->stream, wl_a, count, minkey,
-                         parent, flags);
-      HDB_LAUNCHED();
-      std::swap(wl_a, wl_b);
-      if (round > 64) return ctx->fail(HDB_EDEVICE, "Boruvka did not converge");
-    }
+    hipLaunchKernelGGL(k_bv_hook, dim3(gv), dim3(256), 0, ctx->stream, n, comp, best_w, best_p, cw, cp, next, out, cap,
+                       cnt + 1);
+    HDB_LAUNCHED();
+    hipLaunchKernelGGL(k_bv_jump, dim3(gv), dim3(256), 0, ctx->stream, n, comp, next);
+    HDB_LAUNCHED();
   }
+  int64_t m = 0;
+  HDB_CALL(read_i64(ctx, reinterpret_cast<const int64_t*>(cnt + 1), &m));
   HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(cnt), 1, 0));
-  hipLaunchKernelGGL(k_count_roots, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, parent, n, cnt);
+  hipLaunchKernelGGL(k_count_roots, dim3(gv), dim3(256), 0, ctx->stream, comp, n, cnt);
   HDB_LAUNCHED();
-  return read_i64(ctx, reinterpret_cast<const int64_t*>(cnt), components_out);
-}
-
-// ------------------------------------------------------- edge extraction --
-// Stable compaction of flagged edges (buildMST, mst.cu:291-334): tiles of
-// 4096 edges, 4 waves x 16 chunks of 64.
-constexpr int kCmpTile = 4096;
-
-__global__ __launch_bounds__(256) void k_flag_count(const uint8_t* __restrict__ flags, int64_t m,
-                                                    int32_t* __restrict__ tilecnt) {
-  __shared__ int32_t ws[4];
-  const int64_t base = static_cast<int64_t>(blockIdx.x) * kCmpTile + (threadIdx.x >> 6) * 1024;
-  int c = 0;
-  for (int ch = 0; ch < 16; ++ch) {
-    const int64_t i = base + ch * 64 + __lane_id();
-    c += __popcll(__ballot(i < m && flags[i]));
-  }
-  if (__lane_id() == 0) ws[threadIdx.x >> 6] = c;
-  __syncthreads();
-  if (threadIdx.x == 0) tilecnt[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
-}
-
-__global__ __launch_bounds__(256) void k_flag_write(const uint8_t* __restrict__ flags, int64_t m,
-                                                    const int64_t* __restrict__ tileoff, const int32_t* __restrict__ src,
-                                                    const int32_t* __restrict__ dst, const float* __restrict__ w,
-                                                    hdb_mst_edge* __restrict__ out, int64_t cap) {
-  __shared__ int32_t ws[4];
-  const unsigned wv = threadIdx.x >> 6;
-  const int64_t base = static_cast<int64_t>(blockIdx.x) * kCmpTile + wv * 1024;
-  int c = 0;
-  for (int ch = 0; ch < 16; ++ch) {
-    const int64_t i = base + ch * 64 + __lane_id();
-    c += __popcll(__ballot(i < m && flags[i]));
-  }
-  if (__lane_id() == 0) ws[wv] = c;
-  __syncthreads();
-  int64_t off = tileoff[blockIdx.x];
-  for (unsigned q = 0; q < wv; ++q) off += ws[q];
-  for (int ch = 0; ch < 16; ++ch) {
-    const int64_t i = base + ch * 64 + __lane_id();
-    const bool f = i < m && flags[i];
-    const uint64_t bal = __ballot(f);
-    if (f) {
-      hdb_mst_edge ed;
-      ed.from_node = src[i];
-      ed.to_node = dst[i];
-      ed.weight = w[i];
-      const int64_t pos = off + __popcll(bal & lanemask_lt());
-      if (pos < cap) out[pos] = ed;
-    }
-    off += __popcll(bal);
-  }
-}
-
-int mst_extract(hdb_ctx* ctx, const int32_t* src, const int32_t* dst, const float* w, const uint8_t* flags,
-                int64_t m, hdb_mst_edge* out, int64_t cap, int64_t* count_out) {
-  Stage st(ctx, "mst_extract");
-  const int64_t tiles = std::max<int64_t>((m + kCmpTile - 1) / kCmpTile, 1);
-  int32_t* tilecnt;
-  int64_t* tileoff;
-  HDB_CALL(scratch_t(ctx, kSlotMisc0, tiles, &tilecnt));
-  HDB_CALL(scratch_t(ctx, kSlotMisc1, tiles + 1, &tileoff));
-  hipLaunchKernelGGL(k_flag_count, dim3(tiles), dim3(256), 0, ctx->stream, flags, m, tilecnt);
-  HDB_LAUNCHED();
-  HDB_CALL(scan_i32_to_i64(ctx, tilecnt, tiles, tileoff));
-  hipLaunchKernelGGL(k_flag_write, dim3(tiles), dim3(256), 0, ctx->stream, flags, m, tileoff, src, dst, w, out, cap);
-  HDB_LAUNCHED();
-  HDB_CALL(read_i64(ctx, tileoff + tiles, count_out));
-  if (*count_out > cap) return ctx->fail(HDB_EDEVICE, "MST edge count exceeds n-1");
-  return HDB_OK;
-}
-
-// ---------------------------------------------------------- CSR -> upper --
-// Entries j of row i with nlist[j] > i, in CSR order (the "n > i" direction
-// of cpuMST, mst.cu:42-50): one wave per row.
-__global__ __launch_bounds__(256) void k_upper_count(const int64_t* __restrict__ nindex, const int32_t* __restrict__ nlist,
-                                                     int64_t n, int32_t* __restrict__ cnt) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
-  if (i >= n) return;
-  int c = 0;
-  for (int64_t j = nindex[i] + __lane_id(); j - __lane_id() < nindex[i + 1]; j += 64) {
-    const bool up = j < nindex[i + 1] && nlist[j] > i && nlist[j] < n;
-    c += __popcll(__ballot(up));
-  }
-  if (__lane_id() == 0) cnt[i] = c;
-}
-
-__global__ __launch_bounds__(256) void k_upper_fill(const int64_t* __restrict__ nindex, const int32_t* __restrict__ nlist,
-                                                    const float* __restrict__ w, int64_t n,
-                                                    const int64_t* __restrict__ off, int32_t* __restrict__ usrc,
-                                                    int32_t* __restrict__ udst, float* __restrict__ uw) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
-  if (i >= n) return;
-  int64_t o = off[i];
-  for (int64_t j = nindex[i] + __lane_id(); j - __lane_id() < nindex[i + 1]; j += 64) {
-    const bool up = j < nindex[i + 1] && nlist[j] > i && nlist[j] < n;
-    const uint64_t bal = __ballot(up);
-    if (up) {
-      const int64_t pos = o + __popcll(bal & lanemask_lt());
-      usrc[pos] = static_cast<int32_t>(i);
-      udst[pos] = nlist[j];
-      uw[pos] = w[j];
-    }
-    o += __popcll(bal);
-  }
-}
-
-int csr_upper(hdb_ctx* ctx, const int64_t* nindex, const int32_t* nlist, const float* w, int64_t n, int64_t E,
-              int32_t* usrc, int32_t* udst, float* uw, int64_t* m_out) {
-  (void)E;
-  Stage st(ctx, "csr_upper");
-  int32_t* cnt;
-  int64_t* off;
-  HDB_CALL(scratch_t(ctx, kSlotMisc0, n, &cnt));
-  HDB_CALL(scratch_t(ctx, kSlotMisc1, n + 1, &off));
-  hipLaunchKernelGGL(k_upper_count, dim3(grid_for(n, 4)), dim3(256), 0, ctx->stream, nindex, nlist, n, cnt);
-  HDB_LAUNCHED();
-  HDB_CALL(scan_i32_to_i64(ctx, cnt, n, off));
-  if (usrc) {
-    hipLaunchKernelGGL(k_upper_fill, dim3(grid_for(n, 4)), dim3(256), 0, ctx->stream, nindex, nlist, w, n, off, usrc,
-                       udst, uw);
+  HDB_CALL(read_i64(ctx, reinterpret_cast<const int64_t*>(cnt), components_out));
+  if (m > cap || m != n - *components_out) return ctx->fail(HDB_EDEVICE, "MST edge count does not match the forest");
+  *m_out = m;
+  if (m > 1) {
+    // buildMST (mst.cu:291-334) lists tree edges in CSR order
+    Stage so(ctx, "mst_csr_order");
+    uint32_t* keys;
+    uint64_t* vals;
+    HDB_CALL(scratch_t(ctx, kSlotMstSrc, m, &keys));
+    HDB_CALL(scratch_t(ctx, kSlotMstPos, m, &vals));
+    const unsigned ge = grid_for(m, 256);
+    const int bits = bits_for(n - 1);
+    hipLaunchKernelGGL(k_csr_split, dim3(ge), dim3(256), 0, ctx->stream, out, m, keys, vals);
+    HDB_LAUNCHED();
+    HDB_CALL(radix_sort_u32_u64(ctx, keys, vals, m, 0, bits));
+    hipLaunchKernelGGL(k_csr_swap, dim3(ge), dim3(256), 0, ctx->stream, keys, vals, m);
+    HDB_LAUNCHED();
+    HDB_CALL(radix_sort_u32_u64(ctx, keys, vals, m, 0, bits));
+    hipLaunchKernelGGL(k_csr_join, dim3(ge), dim3(256), 0, ctx->stream, keys, vals, m, out);
     HDB_LAUNCHED();
   }
-  return read_i64(ctx, off + n, m_out);
+  return HDB_OK;
 }
 
 // ------------------------------------------------------------ edge sort --

@@ -687,7 +687,8 @@ __global__ void k_sum_parts(const unsigned long long* __restrict__ part, int64_t
 // --------------------------------------------------------- unique merge --
 // UpdateKNNGraphKernel / UniqueMergeSequential (gpuknn/gen_large_knngraph.cu:
 // 136-213) with PreProcIDKernel (:215-228): running list = first 32 distinct
-// of merge(running, pair result remapped to global ids).
+// of merge(running, pair result remapped to global ids).  Entries are written
+// with plain (non-negative) ids: these lists are the exported result.
 __global__ __launch_bounds__(256) void k_unique_merge(hdb_nnd_element* __restrict__ running,
                                                       const hdb_nnd_element* __restrict__ pair, int64_t n_first,
                                                       int64_t rows, int64_t row0_pair, int64_t off_a,
@@ -699,7 +700,7 @@ __global__ __launch_bounds__(256) void k_unique_merge(hdb_nnd_element* __restric
   if (lane < 32) {
     const hdb_nnd_element e = running[r * kK + lane];
     const uint32_t id = label_id(e.label);
-    key = id == kInvalidId ? kKeyMax : make_key(e.distance, id, 0u);
+    key = id == kInvalidId ? kKeyMax : make_key(e.distance, id, 1u);
   } else {
     const hdb_nnd_element e = pair[(row0_pair + r) * kK + (lane - 32)];
     uint32_t id = label_id(e.label);
@@ -707,7 +708,7 @@ __global__ __launch_bounds__(256) void k_unique_merge(hdb_nnd_element* __restric
       key = kKeyMax;
     } else {
       const int64_t g = (static_cast<int64_t>(id) < n_first) ? off_a + id : off_b + (id - n_first);
-      key = make_key(e.distance, static_cast<uint32_t>(g), 0u);
+      key = make_key(e.distance, static_cast<uint32_t>(g), 1u);
     }
   }
   key = bitonic_group<64>(key);
