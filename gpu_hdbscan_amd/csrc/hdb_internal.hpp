@@ -28,7 +28,7 @@ enum Slot : int {
   kSlotScanTmp, kSlotRowCnt, kSlotRowCursor, kSlotRowTmp, kSlotRowUniq,  // CSR build
   kSlotAhFlag, kSlotAhAux, kSlotAhKeys, kSlotAhKeys2,                 // anti-hubs
   kSlotMstParent, kSlotMstMin, kSlotMstWl1, kSlotMstWl2, kSlotMstFlag, kSlotMstCnt,
-  kSlotMstSrc, kSlotMstPos, kSlotMstCw, kSlotSeedX,
+  kSlotMstSrc, kSlotMstPos, kSlotMstCw, kSlotSeedX, kSlotSeedCand,
   kSlotSortKeys, kSlotSortVals, kSlotSortHist, kSlotSortAux,          // radix sort
   kSlotPipeNindex, kSlotPipeNlist, kSlotPipeW, kSlotPipeAh,  // pipeline
   kSlotBfTmp, kSlotMisc0, kSlotMisc1, kSlotMisc2,

@@ -340,15 +340,26 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
   p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
 }
 
-// Sort keys of the bucket order: (12-bit code << 20) | 20-bit rank of x.u
-// (u a fixed Gaussian direction, x.u clamped to [-lim, lim]).
-__global__ void k_mlp_keys(const float* __restrict__ Z, const float* __restrict__ X, int d,
-                           const float* __restrict__ u, float lim, int64_t n, uint32_t* __restrict__ keys,
+// Sort keys of one bucket table: (12-bit code << 20) | 20-bit rank of x.u.
+// The code is the sign pattern of the 12-d embedding z, rotated by R
+// (12 x 12, row-major; null = identity, the trained bits); u is a fixed unit
+// direction and x.u is clamped to [-lim, lim].
+__global__ void k_mlp_keys(const float* __restrict__ Z, const float* __restrict__ R, const float* __restrict__ X,
+                           int d, const float* __restrict__ u, float lim, int64_t n, uint32_t* __restrict__ keys,
                            uint32_t* __restrict__ ids) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  float z[kCode];
+  for (int b = 0; b < kCode; ++b) z[b] = Z[i * kOutP + b];
   uint32_t code = 0;
-  for (int b = 0; b < kCode; ++b) code |= (Z[i * kOutP + b] > 0.0f ? 1u : 0u) << b;
+  for (int b = 0; b < kCode; ++b) {
+    float v = z[b];
+    if (R) {
+      v = 0.0f;
+      for (int c = 0; c < kCode; ++c) v = __builtin_fmaf(z[c], R[c * kCode + b], v);
+    }
+    code |= (v > 0.0f ? 1u : 0u) << b;
+  }
   float s = 0.0f;
   for (int k = 0; k < d; ++k) s = __builtin_fmaf(X[i * d + k], u[k], s);
   float t = (s / lim) * 0.5f + 0.5f;
@@ -368,10 +379,12 @@ __global__ void k_mlp_permute(const float* __restrict__ X, const uint32_t* __res
   Xs[t] = X[static_cast<int64_t>(order[r]) * d + k];
 }
 
-constexpr int kSeedWin = 64;   // bucket-order neighbours examined per point
-constexpr int kSeedKeep = 24;  // nearest of them kept; the other 8 entries are random ids
-constexpr int kSeedPts = 32;   // points per block (4 waves x 8)
+constexpr int kSeedWin = 128;   // bucket-order neighbours examined per point and table
+constexpr int kSeedTables = 4;  // bucket tables (table 0: the trained code bits)
+constexpr int kSeedKeep = 16;   // nearest window entries kept per table
+constexpr int kSeedPts = 32;    // points per block (4 waves x 8)
 constexpr int kSeedRows = kSeedPts + kSeedWin;
+static_assert(kSeedTables * kSeedKeep == 64, "one candidate per lane in the merge");
 
 __device__ __forceinline__ int64_t seed_lo(int64_t r, int64_t n) {
   int64_t lo = r - kSeedWin / 2;
@@ -379,47 +392,71 @@ __device__ __forceinline__ int64_t seed_lo(int64_t r, int64_t n) {
   return lo < 0 ? 0 : lo;
 }
 
-// Initial lists from the bucket order: the point at sorted position r takes
-// the kSeedKeep nearest of the kSeedWin points around it in that order
-// (exact squared distances, rows staged in LDS), plus 32 - kSeedKeep random
-// ids that keep the graph connected across buckets.
+// One table: the point at sorted position r keeps the kSeedKeep nearest of
+// the kSeedWin points around it in the table's order (exact squared
+// distances, rows staged in LDS) as keys in cand[self][table][.].
+// kStaged: the block's rows sit in LDS (kSeedRows x (d + 1) floats);
+// otherwise they are read from Xs, where they are contiguous.
+template <bool kStaged>
 __global__ __launch_bounds__(256) void k_mlp_seed(const float* __restrict__ Xs, const uint32_t* __restrict__ order,
-                                                  int64_t n, int d, uint64_t seed,
-                                                  hdb_nnd_element* __restrict__ lists) {
-  extern __shared__ float rows[];  // kSeedRows x (d + 1)
-  const int dp = d + 1;
+                                                  int64_t n, int d, int table, uint64_t* __restrict__ cand) {
+  extern __shared__ float lds_rows[];
   const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kSeedPts;
   const int64_t sb = seed_lo(r0, n);
-  int64_t se = seed_lo(r0 + kSeedPts - 1, n) + kSeedWin + 1;
-  if (se > n) se = n;
-  for (int64_t t = threadIdx.x; t < (se - sb) * d; t += blockDim.x) {
-    const int64_t rr = t / d;
-    rows[rr * dp + (t - rr * d)] = Xs[sb * d + t];
+  const float* rows = Xs + sb * d;
+  int dp = d;
+  if (kStaged) {
+    int64_t se = seed_lo(r0 + kSeedPts - 1, n) + kSeedWin + 1;
+    if (se > n) se = n;
+    dp = d + 1;
+    for (int64_t t = threadIdx.x; t < (se - sb) * d; t += blockDim.x) {
+      const int64_t rr = t / d;
+      lds_rows[rr * dp + (t - rr * d)] = Xs[sb * d + t];
+    }
+    __syncthreads();
+    rows = lds_rows;
   }
-  __syncthreads();
   const unsigned lane = __lane_id();
   const int wv = static_cast<int>(threadIdx.x >> 6);
   for (int q = 0; q < kSeedPts / 4; ++q) {
     const int64_t r = r0 + wv * (kSeedPts / 4) + q;
     if (r >= n) break;  // uniform across the wave
     const int64_t lo = seed_lo(r, n);
-    int64_t p = lo + lane;
-    if (p >= r) p += 1;
-    uint64_t key = kKeyMax;
-    if (p < n) key = make_key(d2_any(rows + (p - sb) * dp, rows + (r - sb) * dp, d), order[p], 1u);
-    key = bitonic_group<64>(key);
-    const uint32_t self = order[r];
-    hdb_nnd_element e;
-    if (lane < static_cast<unsigned>(kSeedKeep)) {
-      e = key == kKeyMax ? hdb_nnd_element{__uint_as_float(0x7F800000u), static_cast<int32_t>(kInvalidId)}
-                         : hdb_nnd_element{key_d2(key), static_cast<int32_t>(key_id(key))};
-    } else {
-      uint64_t x = xorshift64star(seed + static_cast<uint64_t>(self) * kK + lane + 1);
-      uint32_t id = static_cast<uint32_t>(x % static_cast<uint64_t>(n));
-      if (id == self) id = static_cast<uint32_t>((id + 1) % static_cast<uint64_t>(n));
-      e = hdb_nnd_element{0.0f, static_cast<int32_t>(id)};
+    const float* self_row = rows + (r - sb) * dp;
+    uint64_t k2[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      int64_t p = lo + h * 64 + lane;
+      if (p >= r) p += 1;
+      k2[h] = p < n ? make_key(d2_any(rows + (p - sb) * dp, self_row, d), order[p], 1u) : kKeyMax;
     }
-    if (lane < static_cast<unsigned>(kK)) lists[static_cast<int64_t>(self) * kK + lane] = e;
+    bitonic128(k2[0], k2[1]);
+    if (lane < static_cast<unsigned>(kSeedKeep))
+      cand[static_cast<int64_t>(order[r]) * (kSeedTables * kSeedKeep) + table * kSeedKeep + lane] = k2[0];
+  }
+}
+
+// Per point: the 32 nearest distinct ids among its table candidates; slots
+// left empty take xorshift ids.
+__global__ __launch_bounds__(256) void k_mlp_merge(const uint64_t* __restrict__ cand, int64_t n, uint64_t seed,
+                                                   hdb_nnd_element* __restrict__ lists) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (p >= n) return;  // uniform across the wave
+  const unsigned lane = __lane_id();
+  uint64_t key = cand[p * 64 + lane];
+  key = bitonic_group<64>(key);
+  const uint64_t prev = shfl_u64(key, (lane + 63) & 63);
+  const bool keep = key != kKeyMax && (lane == 0 || (prev >> 1) != (key >> 1));
+  const uint64_t bal = __ballot(keep);
+  const int pos = __popcll(bal & lanemask_lt());
+  const int total = __popcll(bal);
+  if (keep && pos < kK)
+    lists[p * kK + pos] = hdb_nnd_element{key_d2(key), static_cast<int32_t>(key_id(key))};
+  if (static_cast<int>(lane) >= total && lane < static_cast<unsigned>(kK)) {
+    uint32_t id = static_cast<uint32_t>(xorshift64star(seed + static_cast<uint64_t>(p) * kK + lane + 1) %
+                                        static_cast<uint64_t>(n));
+    if (id == static_cast<uint32_t>(p)) id = static_cast<uint32_t>((id + 1) % static_cast<uint64_t>(n));
+    lists[p * kK + lane] = hdb_nnd_element{0.0f, static_cast<int32_t>(id)};
   }
 }
 
@@ -664,39 +701,64 @@ int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* net, const float* X, int64_t n, int3
                      hdb_nnd_element* lists) {
   if (!ctx || !net || !X || !lists || n < 34 || d != net->L.kin)
     return ctx ? ctx->fail(HDB_EINVAL, "bad learned_init args (n must exceed 33)") : 1;
+  if (n >= (1ll << 31)) return ctx->fail(HDB_EUNSUPPORTED, "more than 2^31 points");
   HDB_HIP(hipSetDevice(ctx->device));
   Stage st(ctx, "learned_init");
-  float *Z, *u;
+  float *Z, *dirs, *Xs;
   uint32_t *keys, *ids;
+  uint64_t* cand;
+  const int64_t per_table = static_cast<int64_t>(kCode) * kCode + d;  // rotation + direction
   HDB_CALL(scratch_t(ctx, kSlotMisc0, n * kOutP, &Z));
   HDB_CALL(scratch_t(ctx, kSlotMisc1, n, &keys));
   HDB_CALL(scratch_t(ctx, kSlotMisc2, n, &ids));
-  HDB_CALL(scratch_t(ctx, kSlotBfTmp, d, &u));
-  std::vector<float> uh(d);
+  HDB_CALL(scratch_t(ctx, kSlotBfTmp, kSeedTables * per_table, &dirs));
+  HDB_CALL(scratch_t(ctx, kSlotSeedX, n * d, &Xs));
+  HDB_CALL(scratch_t(ctx, kSlotSeedCand, n * kSeedTables * kSeedKeep, &cand));
+  // per table: a random rotation of the code space (table 0: none) and a
+  // unit direction for the in-bucket order, Box-Muller from xorshift64*
+  std::vector<float> hd(static_cast<size_t>(kSeedTables * per_table));
   uint64_t s = seed * 0x2545F4914F6CDD1Dull + 7;
-  double nrm = 0.0;
-  for (int k = 0; k < d; ++k) {  // Box-Muller Gaussian direction
+  auto gauss = [&s]() {
     s = xorshift64star(s);
     const double u1 = ((s >> 11) + 1.0) / 9007199254740993.0;
     s = xorshift64star(s);
     const double u2 = (s >> 11) / 9007199254740992.0;
-    uh[k] = static_cast<float>(std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2));
-    nrm += static_cast<double>(uh[k]) * uh[k];
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+  };
+  for (int t = 0; t < kSeedTables; ++t) {
+    float* R = hd.data() + t * per_table;
+    for (int k = 0; k < kCode * kCode; ++k) R[k] = static_cast<float>(gauss());
+    float* u = R + kCode * kCode;
+    double nrm = 0.0;
+    for (int k = 0; k < d; ++k) {
+      u[k] = static_cast<float>(gauss());
+      nrm += static_cast<double>(u[k]) * u[k];
+    }
+    for (int k = 0; k < d; ++k) u[k] = static_cast<float>(u[k] / std::sqrt(nrm));
   }
-  for (int k = 0; k < d; ++k) uh[k] = static_cast<float>(uh[k] / std::sqrt(nrm));
-  HDB_HIP(hipMemcpyAsync(u, uh.data(), d * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  HDB_HIP(hipMemcpyAsync(dirs, hd.data(), hd.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
   HDB_CALL(mlp_forward(ctx, net, X, d, nullptr, n, nullptr, nullptr, nullptr, nullptr, Z));
-  // x.u range: 4 standard deviations of a unit direction over typical data
-  hipLaunchKernelGGL(k_mlp_keys, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, Z, X, d, u, 64.0f, n, keys, ids);
-  HDB_LAUNCHED();
-  HDB_CALL(radix_sort_u32_u32(ctx, keys, ids, n, 0, 32));
-  float* Xs;
-  HDB_CALL(scratch_t(ctx, kSlotSeedX, n * d, &Xs));
-  hipLaunchKernelGGL(k_mlp_permute, dim3(grid_for(n * d, 256)), dim3(256), 0, ctx->stream, X, ids, n, d, Xs);
-  HDB_LAUNCHED();
   const size_t lds = static_cast<size_t>(kSeedRows) * (d + 1) * sizeof(float);
-  hipLaunchKernelGGL(k_mlp_seed, dim3(grid_for(n, kSeedPts)), dim3(256), lds, ctx->stream, Xs, ids, n, d, seed,
-                     lists);
+  const bool staged = lds <= 64 * 1024;
+  for (int t = 0; t < kSeedTables; ++t) {
+    const float* R = t == 0 ? nullptr : dirs + t * per_table;
+    const float* u = dirs + t * per_table + kCode * kCode;
+    // x.u range: 64 covers the spread of a unit direction over typical data
+    hipLaunchKernelGGL(k_mlp_keys, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, Z, R, X, d, u, 64.0f, n, keys,
+                       ids);
+    HDB_LAUNCHED();
+    HDB_CALL(radix_sort_u32_u32(ctx, keys, ids, n, 0, 32));
+    hipLaunchKernelGGL(k_mlp_permute, dim3(grid_for(n * d, 256)), dim3(256), 0, ctx->stream, X, ids, n, d, Xs);
+    HDB_LAUNCHED();
+    if (staged)
+      hipLaunchKernelGGL(k_mlp_seed<true>, dim3(grid_for(n, kSeedPts)), dim3(256), lds, ctx->stream, Xs, ids, n, d, t,
+                         cand);
+    else
+      hipLaunchKernelGGL(k_mlp_seed<false>, dim3(grid_for(n, kSeedPts)), dim3(256), 0, ctx->stream, Xs, ids, n, d, t,
+                         cand);
+    HDB_LAUNCHED();
+  }
+  hipLaunchKernelGGL(k_mlp_merge, dim3(grid_for(n, 4)), dim3(256), 0, ctx->stream, cand, n, seed, lists);
   HDB_LAUNCHED();
   return HDB_OK;
 }

@@ -58,3 +58,13 @@ def test_training_and_learned_init(ctx):
     assert r_trained > r_random
     res = learned.nndescent_learned(ctx, Xd, net, iterations=4, target=0.9)
     print("learned 4 its sampled recall", res.sampled_recall, "fallback", res.fell_back, flush=True)
+    assert not res.fell_back and res.sampled_recall >= 0.9
+    # the gain: at the same iteration count, learned initial lists end far
+    # closer to the exact graph than random ones
+    Lp = api.NNDescent(ctx, Xd, iteration=3, store_result_in_device=True)
+    L3 = net.init_lists(Xd)
+    api.NNDescentFrom(ctx, Xd, L3, 3, store_result_in_device=True)
+    r_plain3 = O.recall(Lp.numpy()["label"][q], gt, 32)
+    r_learned3 = O.recall(L3.numpy()["label"][q], gt, 32)
+    print("3 iterations: random init", r_plain3, "learned init", r_learned3, flush=True)
+    assert r_learned3 >= r_plain3 + 0.1
