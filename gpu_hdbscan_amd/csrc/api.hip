@@ -294,7 +294,7 @@ int hdb_mrd_graph(hdb_ctx* ctx, const float* X, const hdb_nnd_element* lists, co
                   float* w, int64_t* edges_out) {
   HDB_CALL(check_ctx(ctx));
   HDB_CALL(set_device(ctx));
-  return mrd_graph(ctx, X, lists, core, ah, h, n, d, k, false, nindex, nlist, w, nullptr, edges_out);
+  return mrd_graph(ctx, X, lists, core, ah, h, n, d, k, nindex, nlist, w, edges_out);
 }
 
 int hdb_mst(hdb_ctx* ctx, const int64_t* nindex, const int32_t* nlist, const float* w, int64_t n,
@@ -360,7 +360,7 @@ int hdb_pipeline(hdb_ctx* ctx, const float* X, int64_t n, int32_t d, const hdb_p
   int64_t E = 0;
   {
     Stage st(ctx, "stage_mrd_graph");
-    HDB_CALL(mrd_graph(ctx, X, lists, core, ah, h, n, d, params->k, false, nindex, nlist, w, nullptr, &E));
+    HDB_CALL(mrd_graph(ctx, X, lists, core, ah, h, n, d, params->k, nindex, nlist, w, &E));
   }
   ctx->csr_edges = E;
   if (csr_edges_out) *csr_edges_out = E;

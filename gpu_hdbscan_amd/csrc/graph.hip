@@ -162,213 +162,139 @@ int core_distances(hdb_ctx* ctx, const float* X, const hdb_nnd_element* lists, i
 }
 
 // ------------------------------------------------------ MRD graph (CSR) --
-// Every kNN entry (i -> v) contributes v to row i and i to row v (full) or
-// only max(i,v) to row min(i,v) (upper).  Rows are then sorted and made
-// unique, which yields N_i u R_i of calculate_nindex / calculate_nlist
-// (calculates.cu:286-409) in neighbour-id order.
-template <bool UPPER>
-__global__ void k_edge_count(const hdb_nnd_element* __restrict__ lists, int64_t n, int k,
-                             int32_t* __restrict__ rowcnt) {
+// Row v of the symmetrised graph is N_v u R_v (calculate_nindex /
+// calculate_nlist, calculates.cu:286-409) in neighbour-id order, followed in
+// id order by the anti-hub clique when v is an anti-hub.  R_v comes from a
+// stable radix sort of (v, i) pairs taken over all lists in (i, slot) order,
+// so each reverse segment is already in ascending source order; no atomics.
+
+// Pair (row v, source i) of every valid entry; invalid entries get row n.
+__global__ void k_rev_emit(const hdb_nnd_element* __restrict__ lists, int64_t n, int k, uint32_t* __restrict__ key,
+                           uint32_t* __restrict__ val) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= n * k) return;
   const int64_t i = t / k;
   const uint32_t v = label_id(lists[t].label);
-  if (v == kInvalidId || static_cast<int64_t>(v) >= n || static_cast<int64_t>(v) == i) return;
-  if (UPPER) {
-    atomicAdd(&rowcnt[min(static_cast<int64_t>(v), i)], 1);
-  } else {
-    atomicAdd(&rowcnt[i], 1);
-    atomicAdd(&rowcnt[v], 1);
-  }
+  const bool ok = v != kInvalidId && static_cast<int64_t>(v) < n && static_cast<int64_t>(v) != i;
+  key[t] = ok ? v : static_cast<uint32_t>(n);
+  val[t] = static_cast<uint32_t>(i);
 }
 
-template <bool UPPER>
-__global__ void k_edge_scatter(const hdb_nnd_element* __restrict__ lists, int64_t n, int k,
-                               const int64_t* __restrict__ rowoff, int32_t* __restrict__ cursor,
-                               int32_t* __restrict__ tmp) {
-  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= n * k) return;
-  const int64_t i = t / k;
-  const uint32_t v = label_id(lists[t].label);
-  if (v == kInvalidId || static_cast<int64_t>(v) >= n || static_cast<int64_t>(v) == i) return;
-  if (UPPER) {
-    const int64_t a = min(static_cast<int64_t>(v), i), bb = max(static_cast<int64_t>(v), i);
-    tmp[rowoff[a] + atomicAdd(&cursor[a], 1)] = static_cast<int32_t>(bb);
-  } else {
-    tmp[rowoff[i] + atomicAdd(&cursor[i], 1)] = static_cast<int32_t>(v);
-    tmp[rowoff[v] + atomicAdd(&cursor[v], 1)] = static_cast<int32_t>(i);
-  }
+__device__ __forceinline__ int64_t clique_share(int64_t v, const int32_t* ahrank, int64_t h) {
+  return ahrank[v] < 0 ? 0 : h - 1;
 }
 
-// Rows of <= 128 entries: one wave per row, 2 keys per lane.  Longer rows
-// are appended to `big` for k_row_sort_big.  Writes the sorted unique ids to
-// the start of the row segment, uniq[v] and total[v] = uniq + clique share.
-constexpr int kSmallRow = 128;
-constexpr int kBigRowMax = 16384;  // 64 KiB of LDS
-
-__device__ __forceinline__ int64_t clique_share(int64_t v, const int32_t* ahrank, int64_t h, bool upper) {
-  const int32_t r = ahrank[v];
-  if (r < 0) return 0;
-  return upper ? (h - 1 - r) : (h - 1);
-}
-
-template <bool UPPER>
-__global__ __launch_bounds__(256) void k_row_sort_small(const int64_t* __restrict__ rowoff, int64_t n,
-                                                        int32_t* __restrict__ tmp, int32_t* __restrict__ uniq,
-                                                        int32_t* __restrict__ total, const int32_t* __restrict__ ahrank,
-                                                        int64_t h, int32_t* __restrict__ big,
-                                                        int32_t* __restrict__ nbig, int32_t* __restrict__ maxlen) {
+// One wave per row: own ids sorted and made unique (own_sorted, uown), the
+// reverse segment compacted in place to the unique ids that are not own ids
+// (urev), and the row length with the clique share (total).
+__global__ __launch_bounds__(256) void k_row_count(const hdb_nnd_element* __restrict__ lists, int64_t n, int k,
+                                                   const int32_t* __restrict__ rs, const int32_t* __restrict__ re,
+                                                   uint32_t* __restrict__ rev, const int32_t* __restrict__ ahrank,
+                                                   int64_t h, int32_t* __restrict__ own_sorted,
+                                                   int32_t* __restrict__ uown, int32_t* __restrict__ urev,
+                                                   int32_t* __restrict__ total) {
   const unsigned lane = __lane_id();
   const int64_t v = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
-  if (v >= n) return;
-  const int64_t beg = rowoff[v];
-  const int64_t len = rowoff[v + 1] - beg;
-  if (len > kSmallRow) {
-    if (lane == 0) {
-      big[atomicAdd(nbig, 1)] = static_cast<int32_t>(v);
-      atomicMax(maxlen, static_cast<int32_t>(min(len, static_cast<int64_t>(0x7FFFFFFF))));
-    }
-    return;
-  }
-  uint64_t r0 = lane < len ? static_cast<uint32_t>(tmp[beg + lane]) : kKeyMax;
-  uint64_t r1 = 64 + lane < len ? static_cast<uint32_t>(tmp[beg + 64 + lane]) : kKeyMax;
-  bitonic128(r0, r1);
-  const uint64_t p0 = shfl_u64(r0, (lane + 63) & 63);
-  const uint64_t q1 = shfl_u64(r1, (lane + 63) & 63);
-  const uint64_t last0 = shfl_u64(r0, 63);
-  const uint64_t p1 = lane == 0 ? last0 : q1;
-  const bool keep0 = r0 != kKeyMax && (lane == 0 || p0 != r0);
-  const bool keep1 = r1 != kKeyMax && p1 != r1;
-  const uint64_t b0 = __ballot(keep0), b1 = __ballot(keep1);
+  if (v >= n) return;  // uniform across the wave
   const uint64_t lt = lanemask_lt();
-  if (keep0) tmp[beg + __popcll(b0 & lt)] = static_cast<int32_t>(r0);
-  if (keep1) tmp[beg + __popcll(b0) + __popcll(b1 & lt)] = static_cast<int32_t>(r1);
+  uint64_t x = kKeyMax;
+  if (static_cast<int>(lane) < k) {
+    const uint32_t id = label_id(lists[v * k + lane].label);
+    if (id != kInvalidId && static_cast<int64_t>(id) < n && static_cast<int64_t>(id) != v) x = id;
+  }
+  x = bitonic_group<64>(x);
+  const uint64_t px = shfl_u64(x, (lane + 63) & 63);
+  const bool ko = x != kKeyMax && (lane == 0 || px != x);
+  const uint64_t bo = __ballot(ko);
+  const int uo = __popcll(bo);
+  if (ko) own_sorted[v * k + __popcll(bo & lt)] = static_cast<int32_t>(x);
+  const int64_t b0 = rs[v], b1 = re[v];
+  int64_t ur = 0;
+  uint32_t carry = 0xFFFFFFFFu;
+  for (int64_t c = b0; c < b1; c += 64) {
+    const int64_t p = c + lane;
+    const uint32_t y = p < b1 ? rev[p] : 0xFFFFFFFFu;
+    uint32_t py = __shfl(y, static_cast<int>((lane + 63) & 63));
+    if (lane == 0) py = carry;
+    bool keep = p < b1 && y != py;
+    // drop ids that are own entries (mutual kNN pairs are listed once)
+    for (uint64_t m = bo; m; m &= m - 1) {
+      const int l = __builtin_ctzll(m);
+      const uint32_t o = static_cast<uint32_t>(shfl_u64(x, l));
+      keep = keep && y != o;
+    }
+    const uint64_t bk = __ballot(keep);
+    if (keep) rev[b0 + ur + __popcll(bk & lt)] = y;
+    ur += __popcll(bk);
+    carry = __shfl(y, 63);
+  }
   if (lane == 0) {
-    const int32_t u = __popcll(b0) + __popcll(b1);
-    uniq[v] = u;
-    total[v] = static_cast<int32_t>(u + clique_share(v, ahrank, h, UPPER));
+    uown[v] = uo;
+    urev[v] = static_cast<int32_t>(ur);
+    total[v] = static_cast<int32_t>(uo + ur + clique_share(v, ahrank, h));
   }
 }
 
-// One 256-thread block per long row: LDS bitonic sort over the next power
-// of two, then unique.
-template <bool UPPER>
-__global__ __launch_bounds__(256) void k_row_sort_big(const int64_t* __restrict__ rowoff, const int32_t* __restrict__ big,
-                                                      int32_t* __restrict__ tmp, int32_t* __restrict__ uniq,
-                                                      int32_t* __restrict__ total, const int32_t* __restrict__ ahrank,
-                                                      int64_t h) {
-  extern __shared__ uint32_t srt[];
-  __shared__ int32_t wsum[4];
-  const int64_t v = big[blockIdx.x];
-  const int64_t beg = rowoff[v];
-  const int len = static_cast<int>(rowoff[v + 1] - beg);
-  int P = 1;
-  while (P < len) P <<= 1;
-  for (int i = threadIdx.x; i < P; i += 256) srt[i] = i < len ? static_cast<uint32_t>(tmp[beg + i]) : 0xFFFFFFFFu;
-  __syncthreads();
-  for (int kk = 2; kk <= P; kk <<= 1) {
-    for (int j = kk >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < P; i += 256) {
-        const int l = i ^ j;
-        if (l > i) {
-          const uint32_t a = srt[i], c = srt[l];
-          const bool up = (i & kk) == 0;
-          if ((a > c) == up) {
-            srt[i] = c;
-            srt[l] = a;
-          }
-        }
-      }
-      __syncthreads();
-    }
+// # of ids < x (LOWER) or <= x in the sorted array a[0, len).
+template <bool LOWER, typename T>
+__device__ __forceinline__ int64_t rank_in(const T* __restrict__ a, int64_t len, uint32_t x) {
+  int64_t lo = 0, hi = len;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    const uint32_t am = static_cast<uint32_t>(a[mid]);
+    if (LOWER ? (am < x) : (am <= x)) lo = mid + 1; else hi = mid;
   }
-  // unique in order: chunked block scan of keep flags
-  int base = 0;
-  for (int start = 0; start < len; start += 256) {
-    const int i = start + threadIdx.x;
-    const bool keep = i < len && (i == 0 || srt[i] != srt[i - 1]);
-    const uint64_t bal = __ballot(keep);
-    const int w = threadIdx.x >> 6;
-    if (__lane_id() == 0) wsum[w] = __popcll(bal);
-    __syncthreads();
-    int off = 0, tot = 0;
-    for (int q = 0; q < 4; ++q) {
-      if (q < w) off += wsum[q];
-      tot += wsum[q];
-    }
-    if (keep) tmp[beg + base + off + __popcll(bal & lanemask_lt())] = static_cast<int32_t>(srt[i]);
-    base += tot;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    uniq[v] = base;
-    total[v] = static_cast<int32_t>(base + clique_share(v, ahrank, h, UPPER));
-  }
+  return lo;
 }
 
-// Fill row v: merge the unique kNN ids with the clique ids (kNN entry first
-// on equal ids) and compute the weights (calculateMRD calculates.cu:61-82;
-// clique entries raised to the Euclidean distance as in
-// calculate_coreDistance_antihubs :412-446).
-template <bool UPPER>
-__global__ __launch_bounds__(256) void k_row_fill(const float* __restrict__ X, int d, const int64_t* __restrict__ rowoff,
-                                                  const int32_t* __restrict__ tmp, const int32_t* __restrict__ uniq,
-                                                  const int64_t* __restrict__ nindex, int64_t n,
-                                                  const float* __restrict__ core, const int32_t* __restrict__ ahrank,
-                                                  const int32_t* __restrict__ ah, int64_t h,
-                                                  int32_t* __restrict__ nlist, float* __restrict__ w,
-                                                  int32_t* __restrict__ src) {
+// Fill row v: kNN ids (own and reverse) and clique ids merged by rank, kNN
+// entry first on equal ids; weights max(core_v, core_x) for kNN entries
+// (calculateMRD calculates.cu:61-82) and max(core_v, core_c, d_vc) for clique
+// entries (calculate_coreDistance_antihubs :412-446).
+__global__ __launch_bounds__(256) void k_row_fill(const float* __restrict__ X, int d, int64_t n, int k,
+                                                  const int32_t* __restrict__ own_sorted,
+                                                  const int32_t* __restrict__ uown, const uint32_t* __restrict__ rev,
+                                                  const int32_t* __restrict__ rs, const int32_t* __restrict__ urev,
+                                                  const int64_t* __restrict__ nindex, const float* __restrict__ core,
+                                                  const int32_t* __restrict__ ahrank, const int32_t* __restrict__ ah,
+                                                  int64_t h, int32_t* __restrict__ nlist, float* __restrict__ w) {
   const unsigned lane = __lane_id();
   const int64_t v = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
   if (v >= n) return;
-  const int64_t beg = rowoff[v];
-  const int u = uniq[v];
+  const int32_t* own = own_sorted + v * k;
+  const int64_t uo = uown[v];
+  const uint32_t* rv = rev + rs[v];
+  const int64_t ur = urev[v];
   const int64_t out0 = nindex[v];
   const float cv = core[v];
   const int32_t r = ahrank[v];
-  // clique slice: ah[c0, c1) without v itself (full mode skips index r)
-  int64_t c0 = 0, c1 = 0;
-  if (r >= 0) {
-    c0 = UPPER ? r + 1 : 0;
-    c1 = h;
-  }
-  const int64_t ncl = (r >= 0) ? (UPPER ? h - 1 - r : h - 1) : 0;
-  // kNN part
-  for (int i = lane; i < u; i += 64) {
-    const int32_t x = tmp[beg + i];
-    int64_t before = 0;
-    if (ncl > 0) {
-      // clique ids < x (lower bound over the slice, minus self if below x)
-      int64_t lo = c0, hi = c1;
-      while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (ah[mid] < x) lo = mid + 1; else hi = mid;
-      }
-      before = lo - c0;
-      if (!UPPER && r < lo) before -= 1;
-    }
-    const int64_t pos = out0 + i + before;
-    nlist[pos] = x;
+  // clique ids < x, without v itself
+  auto clique_before = [&](uint32_t x) -> int64_t {
+    if (r < 0) return 0;
+    const int64_t lo = rank_in<true>(ah, h, x);
+    return lo - (r < lo ? 1 : 0);
+  };
+  for (int64_t a = lane; a < uo; a += 64) {
+    const uint32_t x = static_cast<uint32_t>(own[a]);
+    const int64_t pos = out0 + a + rank_in<true>(rv, ur, x) + clique_before(x);
+    nlist[pos] = static_cast<int32_t>(x);
     w[pos] = fmaxf(cv, core[x]);
-    if (UPPER) src[pos] = static_cast<int32_t>(v);
   }
-  // clique part
-  for (int64_t q = c0 + lane; q < c1; q += 64) {
-    if (!UPPER && q == r) continue;
-    const int32_t c = ah[q];
-    // kNN ids <= c (upper bound)
-    int lo = 0, hi = u;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (tmp[beg + mid] <= c) lo = mid + 1; else hi = mid;
+  for (int64_t b = lane; b < ur; b += 64) {
+    const uint32_t y = rv[b];
+    const int64_t pos = out0 + b + rank_in<true>(own, uo, y) + clique_before(y);
+    nlist[pos] = static_cast<int32_t>(y);
+    w[pos] = fmaxf(cv, core[y]);
+  }
+  if (r >= 0) {
+    for (int64_t q = lane; q < h; q += 64) {
+      if (q == r) continue;
+      const uint32_t c = static_cast<uint32_t>(ah[q]);
+      const int64_t pos = out0 + (q > r ? q - 1 : q) + rank_in<false>(own, uo, c) + rank_in<false>(rv, ur, c);
+      nlist[pos] = static_cast<int32_t>(c);
+      const float dab = dist_fp64(X + v * d, X + static_cast<int64_t>(c) * d, d);
+      w[pos] = fmaxf(fmaxf(cv, core[c]), dab);
     }
-    int64_t qi = q - c0;
-    if (!UPPER && q > r) qi -= 1;
-    const int64_t pos = out0 + qi + lo;
-    nlist[pos] = c;
-    const float dab = dist_fp64(X + v * d, X + static_cast<int64_t>(c) * d, d);
-    w[pos] = fmaxf(fmaxf(cv, core[c]), dab);
-    if (UPPER) src[pos] = static_cast<int32_t>(v);
   }
 }
 
@@ -378,78 +304,43 @@ __global__ void k_ah_rank(const int32_t* __restrict__ ah, int64_t h, int32_t* __
 }
 
 int mrd_graph(hdb_ctx* ctx, const float* X, const hdb_nnd_element* lists, const float* core, const int32_t* ah,
-              int64_t h, int64_t n, int d, int k, bool upper, int64_t* nindex, int32_t* nlist, float* w, int32_t* src,
-              int64_t* edges_out) {
+              int64_t h, int64_t n, int d, int k, int64_t* nindex, int32_t* nlist, float* w, int64_t* edges_out) {
   if (!X || !lists || !core || !nindex || !edges_out || n < 1 || d < 1 || k < 1 || h < 0 || (h > 0 && !ah))
     return ctx->fail(HDB_EINVAL, "bad mrd_graph arguments");
-  Stage st(ctx, upper ? "mrd_graph_upper" : "mrd_graph_full");
-  const int64_t cap = (upper ? 1 : 2) * n * k;
-  int32_t *rowcnt, *cursor, *tmp, *uniq, *ahrank, *misc;
-  int64_t* rowoff;
-  HDB_CALL(scratch_t(ctx, kSlotRowCnt, n + 1, &rowcnt));
-  HDB_CALL(scratch_t(ctx, kSlotRowCursor, n + 2, &cursor));
-  HDB_CALL(scratch_t(ctx, kSlotRowTmp, std::max<int64_t>(cap, 1), &tmp));
-  HDB_CALL(scratch_t(ctx, kSlotRowUniq, 2 * n + 2, &uniq));
-  HDB_CALL(scratch_t(ctx, kSlotMisc1, n + 1, &rowoff));
-  HDB_CALL(scratch_t(ctx, kSlotAhFlag, n + 8, &ahrank));
-  HDB_CALL(scratch_t(ctx, kSlotMisc2, n + 8, &misc));
-  int32_t* total = uniq + n + 1;
-  int32_t* big = misc;     // row ids of long rows (n)
-  int32_t* nbig = cursor + n;  // two spare cursor slots: long-row count, longest row
-  int32_t* maxlen = cursor + n + 1;
-  HDB_CALL(fill_i32(ctx, rowcnt, n + 1, 0));
-  HDB_CALL(fill_i32(ctx, cursor, n + 1, 0));
+  if (k > 64) return ctx->fail(HDB_EUNSUPPORTED, "lists longer than 64 entries");
+  const int64_t m = n * k;
+  if (m >= (1ll << 31) || n >= (1ll << 31) - 1) return ctx->fail(HDB_EUNSUPPORTED, "more than 2^31 list entries");
+  Stage st(ctx, "mrd_graph");
+  uint32_t *key, *rev;
+  int32_t *own_sorted, *cnt, *ahrank;
+  HDB_CALL(scratch_t(ctx, kSlotRowCursor, m, &key));
+  HDB_CALL(scratch_t(ctx, kSlotRowTmp, m, &rev));
+  HDB_CALL(scratch_t(ctx, kSlotRowUniq, m, &own_sorted));
+  HDB_CALL(scratch_t(ctx, kSlotRowCnt, 5 * n, &cnt));
+  HDB_CALL(scratch_t(ctx, kSlotAhFlag, n, &ahrank));
+  int32_t* uown = cnt;
+  int32_t* urev = cnt + n;
+  int32_t* total = cnt + 2 * n;
+  int32_t* rs = cnt + 3 * n;
+  int32_t* re = cnt + 4 * n;
   HDB_CALL(fill_i32(ctx, ahrank, n, -1));
   if (h > 0) {
     hipLaunchKernelGGL(k_ah_rank, dim3(grid_for(h, 256)), dim3(256), 0, ctx->stream, ah, h, ahrank);
     HDB_LAUNCHED();
   }
-  const unsigned g = grid_for(n * k, 256);
-  if (upper) hipLaunchKernelGGL(k_edge_count<true>, dim3(g), dim3(256), 0, ctx->stream, lists, n, k, rowcnt);
-  else hipLaunchKernelGGL(k_edge_count<false>, dim3(g), dim3(256), 0, ctx->stream, lists, n, k, rowcnt);
+  hipLaunchKernelGGL(k_rev_emit, dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream, lists, n, k, key, rev);
   HDB_LAUNCHED();
-  HDB_CALL(scan_i32_to_i64(ctx, rowcnt, n, rowoff));
-  if (upper)
-    hipLaunchKernelGGL(k_edge_scatter<true>, dim3(g), dim3(256), 0, ctx->stream, lists, n, k, rowoff, cursor, tmp);
-  else
-    hipLaunchKernelGGL(k_edge_scatter<false>, dim3(g), dim3(256), 0, ctx->stream, lists, n, k, rowoff, cursor, tmp);
+  HDB_CALL(radix_sort_u32_u32(ctx, key, rev, m, 0, bits_for(n)));
+  HDB_CALL(segment_bounds(ctx, key, m, n, rs, re));
+  hipLaunchKernelGGL(k_row_count, dim3(grid_for(n, 4)), dim3(256), 0, ctx->stream, lists, n, k, rs, re, rev, ahrank,
+                     h, own_sorted, uown, urev, total);
   HDB_LAUNCHED();
-  HDB_CALL(fill_i32(ctx, nbig, 2, 0));
-  if (upper)
-    hipLaunchKernelGGL(k_row_sort_small<true>, dim3(grid_for(n, 4)), dim3(256), 0, ctx->stream, rowoff, n, tmp, uniq,
-                       total, ahrank, h, big, nbig, maxlen);
-  else
-    hipLaunchKernelGGL(k_row_sort_small<false>, dim3(grid_for(n, 4)), dim3(256), 0, ctx->stream, rowoff, n, tmp, uniq,
-                       total, ahrank, h, big, nbig, maxlen);
-  HDB_LAUNCHED();
-  int32_t nbl[2] = {0, 0};
-  HDB_HIP(hipMemcpyAsync(nbl, nbig, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-  HDB_HIP(hipStreamSynchronize(ctx->stream));
-  const int32_t nb = nbl[0];
-  if (nb > 0) {
-    // the longest row decides the LDS size
-    const int64_t longest = nbl[1];
-    if (longest > kBigRowMax) return ctx->fail(HDB_EUNSUPPORTED, "vertex degree above 16384 in the symmetrised graph");
-    int64_t P = 1;
-    while (P < longest) P <<= 1;
-    if (upper)
-      hipLaunchKernelGGL(k_row_sort_big<true>, dim3(nb), dim3(256), P * sizeof(uint32_t), ctx->stream, rowoff, big,
-                         tmp, uniq, total, ahrank, h);
-    else
-      hipLaunchKernelGGL(k_row_sort_big<false>, dim3(nb), dim3(256), P * sizeof(uint32_t), ctx->stream, rowoff, big,
-                         tmp, uniq, total, ahrank, h);
-    HDB_LAUNCHED();
-  }
   HDB_CALL(scan_i32_to_i64(ctx, total, n, nindex));
   HDB_CALL(read_i64(ctx, nindex + n, edges_out));
   if (!nlist) return HDB_OK;  // size query
-  if (!w || (upper && !src)) return ctx->fail(HDB_EINVAL, "null output array");
-  if (upper)
-    hipLaunchKernelGGL(k_row_fill<true>, dim3(grid_for(n, 4)), dim3(256), 0, ctx->stream, X, d, rowoff, tmp, uniq,
-                       nindex, n, core, ahrank, ah, h, nlist, w, src);
-  else
-    hipLaunchKernelGGL(k_row_fill<false>, dim3(grid_for(n, 4)), dim3(256), 0, ctx->stream, X, d, rowoff, tmp, uniq,
-                       nindex, n, core, ahrank, ah, h, nlist, w, src);
+  if (!w) return ctx->fail(HDB_EINVAL, "null output array");
+  hipLaunchKernelGGL(k_row_fill, dim3(grid_for(n, 4)), dim3(256), 0, ctx->stream, X, d, n, k, own_sorted, uown, rev, rs,
+                     urev, nindex, core, ahrank, ah, h, nlist, w);
   HDB_LAUNCHED();
   return HDB_OK;
 }

@@ -288,11 +288,11 @@ int antihubs(hdb_ctx* ctx, const hdb_nnd_element* lists, const int32_t* counts, 
              int32_t* ids, int64_t* h_out);
 int core_distances(hdb_ctx* ctx, const float* X, const hdb_nnd_element* lists, int64_t n, int d,
                    int k, int mpts, float* core);
-// upper != 0: rows hold only neighbours with larger id (one entry per
-// undirected edge) and src receives each entry's row; upper == 0: full CSR.
+// Full (symmetric) MRD CSR, rows in neighbour-id order.  nlist == nullptr:
+// size query (nindex and *edges_out only).
 int mrd_graph(hdb_ctx* ctx, const float* X, const hdb_nnd_element* lists, const float* core,
-              const int32_t* ah, int64_t h, int64_t n, int d, int k, bool upper,
-              int64_t* nindex, int32_t* nlist, float* w, int32_t* src, int64_t* edges_out);
+              const int32_t* ah, int64_t h, int64_t n, int d, int k,
+              int64_t* nindex, int32_t* nlist, float* w, int64_t* edges_out);
 
 // mst.hip
 // Minimum spanning forest of a full (symmetric) CSR under the order

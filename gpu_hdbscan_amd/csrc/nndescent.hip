@@ -489,6 +489,116 @@ __global__ __launch_bounds__(kJoinWaves * 64) void k_nnd_join(const float* __res
   join_emit<WPB, kSlots>(b, top, active, my_id, nc, rev_drop, s_wcnt, s_new, &s_base);
 }
 
+template <int SLOTS>
+__device__ __forceinline__ void top_insert32(uint32_t (&top)[SLOTS], uint32_t key) {
+#pragma unroll
+  for (int s = 0; s < SLOTS; ++s) {
+    const uint32_t a = top[s];
+    top[s] = a < key ? a : key;
+    key = a < key ? key : a;
+  }
+}
+
+// Partner-split FP32 join for D <= 32.  Phase 1: every candidate computes its
+// squared distance to each new candidate (so each old-new pair is computed
+// once) and old candidates store theirs in an LDS block; phase 2: new
+// candidates read their old partners from that block.  Ranking uses 32-bit
+// keys -- the distance bits with the low 6 bits replaced by the partner's
+// index -- and the KEEP winners' exact squared distances are recomputed from
+// the same rows in the same operation order, so proposals carry exactly the
+// distances of k_nnd_join (pairings differ only among partners whose
+// distances agree to within 2^-17 relative).
+template <int D, bool NEWNEW, int KEEP>
+__global__ __launch_bounds__(kJoinWaves * 64) void k_nnd_join2(const float* __restrict__ X, int64_t n, int d, int S,
+                                                               NndBufs b) {
+  static_assert(D > 0 && D <= 32, "register-resident rows only");
+  constexpr int WPB = kJoinWaves;
+  constexpr int kStride = (D + 3) & ~3;
+  constexpr int kSlots = KEEP == 0 ? 2 : KEEP;
+  constexpr int kBlock = 32 * 33;  // max of nn * (no + 1) over nn + no <= 64
+  constexpr uint32_t kNone = 0xFFFFFFFFu;
+  __shared__ float s_vec[WPB][64 * kStride];
+  __shared__ float s_blk[WPB][kBlock];
+  __shared__ uint32_t s_new[WPB][64];
+  __shared__ uint32_t s_old[WPB][64];
+  __shared__ uint32_t s_cand[WPB][64];
+  __shared__ int32_t s_wcnt[WPB];
+  __shared__ unsigned long long s_base;
+  (void)d;
+  const unsigned lane = __lane_id();
+  const unsigned w = threadIdx.x >> 6;
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * WPB + w;
+
+  int nn, nc, rev_drop;
+  join_candidates<WPB>(b, p, p < n, S, s_new, s_old, s_cand, &nn, &nc, &rev_drop);
+  const int no = nc - nn;
+  const bool active = lane < static_cast<unsigned>(nc);
+  const bool is_new = lane < static_cast<unsigned>(nn);
+  const uint32_t my_id = active ? s_cand[w][lane] : 0u;
+  float own[kStride];
+  if (active) {
+    const float* src = X + static_cast<int64_t>(my_id) * D;
+#pragma unroll
+    for (int k = 0; k < D; ++k) own[k] = src[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < D; ++k) own[k] = 0.0f;
+  }
+#pragma unroll
+  for (int k = D; k < kStride; ++k) own[k] = 0.0f;
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < kStride; ++k) s_vec[w][lane * kStride + k] = own[k];
+  }
+  __syncthreads();
+
+  const int bst = no + 1;  // row stride of the new x old block
+  uint32_t top[kSlots];
+#pragma unroll
+  for (int s = 0; s < kSlots; ++s) top[s] = kNone;
+  // phase 1: partners among the new candidates
+  for (int j = 0; j < nn; ++j) {
+    const float* row = s_vec[w] + j * kStride;
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const float diff = own[k] - row[k];
+      acc = __builtin_fmaf(diff, diff, acc);
+    }
+    if (active && !is_new) s_blk[w][j * bst + (static_cast<int>(lane) - nn)] = acc;
+    const bool ok = active && j != static_cast<int>(lane) && (NEWNEW || !is_new);
+    const uint32_t key = ok ? ((__float_as_uint(acc) & ~63u) | static_cast<uint32_t>(j)) : kNone;
+    if (KEEP == 0) top[0] = key < top[0] ? key : top[0];
+    else top_insert32<kSlots>(top, key);
+  }
+  __syncthreads();
+  // phase 2: new candidates take their old partners from the block
+  for (int o = 0; o < no; ++o) {
+    const float acc = is_new ? s_blk[w][lane * bst + o] : 0.0f;
+    const uint32_t key = is_new ? ((__float_as_uint(acc) & ~63u) | static_cast<uint32_t>(nn + o)) : kNone;
+    if (KEEP == 0) top[1] = key < top[1] ? key : top[1];
+    else top_insert32<kSlots>(top, key);
+  }
+  // exact keys of the winners
+  uint64_t top64[kSlots];
+#pragma unroll
+  for (int s = 0; s < kSlots; ++s) {
+    top64[s] = kKeyMax;
+    if (top[s] != kNone) {
+      const int j = static_cast<int>(top[s] & 63u);
+      const float* row = s_vec[w] + j * kStride;
+      float acc = 0.0f;
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        const float diff = own[k] - row[k];
+        acc = __builtin_fmaf(diff, diff, acc);
+      }
+      top64[s] = make_key(acc, s_cand[w][j], 1u);
+    }
+  }
+  join_emit<WPB, kSlots>(b, top64, active, my_id, nc, rev_drop, s_wcnt, s_new, &s_base);
+}
+
 // BF16 join (distance_mode 1): the 64 x 64 Gram matrix of the candidates is
 // computed with v_mfma_f32_16x16x32_bf16 (BF16 inputs, FP32 accumulation)
 // from the FP32 rows staged in LDS; approximate squared distances
@@ -781,6 +891,12 @@ static int read_stats(hdb_ctx* ctx, int64_t n, const NndBufs& b) {
 template <int D, bool NEWNEW, int KEEP>
 static int launch_join_k(hdb_ctx* ctx, const float* X, int64_t n, int d, int S, const NndBufs& b) {
   constexpr bool kOwnReg = (D > 0 && D <= 32);
+  if (kOwnReg) {
+    hipLaunchKernelGGL((k_nnd_join2<kOwnReg ? D : 12, NEWNEW, KEEP>), dim3(grid_for(n, kJoinWaves)),
+                       dim3(kJoinWaves * 64), 0, ctx->stream, X, n, d, S, b);
+    HDB_LAUNCHED();
+    return HDB_OK;
+  }
   const int stride = kOwnReg ? ((D + 3) & ~3) : d + 1;
   const size_t lds = static_cast<size_t>(kJoinWaves) * 64 * stride * sizeof(float);
   hipLaunchKernelGGL((k_nnd_join<D, NEWNEW, KEEP>), dim3(grid_for(n, kJoinWaves)), dim3(kJoinWaves * 64), lds,

@@ -174,8 +174,13 @@ __global__ __launch_bounds__(kRsBlock) void k_rs_scatter(const K* __restrict__ k
                                                          K* __restrict__ kout, V* __restrict__ vout, int64_t n,
                                                          int shift, int64_t ntiles,
                                                          const int64_t* __restrict__ hoff) {
+  static_assert(kRsBlock == kRsBins, "one thread per digit");
   __shared__ int32_t wcnt[kRsBlock / 64][kRsBins];
   __shared__ int64_t goff[kRsBins];
+  __shared__ int32_t dstart[kRsBins];
+  __shared__ int32_t wsum[kRsBlock / 64];
+  __shared__ K skey[kRsTile];
+  __shared__ V sval[HAS_V ? kRsTile : 1];
   const unsigned lane = __lane_id();
   const unsigned w = threadIdx.x >> 6;
   for (int i = threadIdx.x; i < (kRsBlock / 64) * kRsBins; i += kRsBlock) (&wcnt[0][0])[i] = 0;
@@ -213,25 +218,48 @@ __global__ __launch_bounds__(kRsBlock) void k_rs_scatter(const K* __restrict__ k
     rank[c] = act ? base + __popcll(m & lt) : 0;
   }
   __syncthreads();
-  // Per digit: exclusive prefix over waves, in wave order.
-  for (int dg = threadIdx.x; dg < kRsBins; dg += kRsBlock) {
-    int32_t acc = 0;
+  // Per digit (one thread each): exclusive prefix over waves in wave order,
+  // then the tile's digit starts, an exclusive scan over digits.
+  const int dgt = static_cast<int>(threadIdx.x);
+  int32_t tot = 0;
 #pragma unroll
-    for (int ww = 0; ww < kRsBlock / 64; ++ww) {
-      int32_t c = wcnt[ww][dg];
-      wcnt[ww][dg] = acc;
-      acc += c;
-    }
+  for (int ww = 0; ww < kRsBlock / 64; ++ww) {
+    const int32_t c = wcnt[ww][dgt];
+    wcnt[ww][dgt] = tot;
+    tot += c;
   }
+  int32_t incl = tot;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int32_t y = __shfl_up(incl, o);
+    if (static_cast<int>(lane) >= o) incl += y;
+  }
+  if (lane == 63) wsum[w] = incl;
   __syncthreads();
+  int32_t wpre = 0;
+  for (unsigned ww = 0; ww < w; ++ww) wpre += wsum[ww];
+  dstart[dgt] = wpre + incl - tot;
+  __syncthreads();
+  // Stage the tile in LDS in digit order, then write each digit's run to
+  // consecutive global positions.
 #pragma unroll
   for (int c = 0; c < kRsChunks; ++c) {
     const int64_t idx = wbase + c * 64 + lane;
     if (idx < n) {
-      const int64_t pos = goff[dig[c]] + wcnt[w][dig[c]] + rank[c];
-      kout[pos] = key[c];
-      if (HAS_V) vout[pos] = val[c];
+      const int32_t lp = dstart[dig[c]] + wcnt[w][dig[c]] + rank[c];
+      skey[lp] = key[c];
+      if (HAS_V) sval[lp] = val[c];
     }
+  }
+  __syncthreads();
+  const int64_t tbase = static_cast<int64_t>(blockIdx.x) * kRsTile;
+  const int32_t cnt = static_cast<int32_t>(n - tbase < kRsTile ? n - tbase : kRsTile);
+  for (int32_t i = static_cast<int32_t>(threadIdx.x); i < cnt; i += kRsBlock) {
+    const K k = skey[i];
+    const unsigned dd = static_cast<unsigned>(k >> shift) & 0xFFu;
+    const int64_t pos = goff[dd] + (i - dstart[dd]);
+    kout[pos] = k;
+    if (HAS_V) vout[pos] = sval[i];
   }
 }
 
