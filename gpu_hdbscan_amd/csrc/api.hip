@@ -342,7 +342,8 @@ int hdb_pipeline(hdb_ctx* ctx, const float* X, int64_t n, int32_t d, const hdb_p
   int64_t h = 0;
   {
     Stage st(ctx, "stage_hubs_antihubs");
-    HDB_CALL(hub_counts(ctx, lists, n, params->k, counts));
+    // in-degrees from the row-sorted reverse pairs the CSR build uses
+    HDB_CALL(mrd_prep(ctx, lists, n, params->k, counts));
     HDB_CALL(antihubs(ctx, lists, counts, n, params->k, ah, &h));
   }
   {
@@ -360,7 +361,7 @@ int hdb_pipeline(hdb_ctx* ctx, const float* X, int64_t n, int32_t d, const hdb_p
   int64_t E = 0;
   {
     Stage st(ctx, "stage_mrd_graph");
-    HDB_CALL(mrd_graph(ctx, X, lists, core, ah, h, n, d, params->k, nindex, nlist, w, &E));
+    HDB_CALL(mrd_rows(ctx, X, lists, core, ah, h, n, d, params->k, nindex, nlist, w, &E));
   }
   ctx->csr_edges = E;
   if (csr_edges_out) *csr_edges_out = E;

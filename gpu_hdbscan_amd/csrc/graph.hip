@@ -168,16 +168,24 @@ int core_distances(hdb_ctx* ctx, const float* X, const hdb_nnd_element* lists, i
 // stable radix sort of (v, i) pairs taken over all lists in (i, slot) order,
 // so each reverse segment is already in ascending source order; no atomics.
 
-// Pair (row v, source i) of every valid entry; invalid entries get row n.
+// Pair (row v, source i) of every entry with a label in [0, n); other
+// entries get row n.  Segment lengths are then the in-degrees of
+// countDegrees (count.cu:36-108); self entries are dropped later, by the row
+// pass.
 __global__ void k_rev_emit(const hdb_nnd_element* __restrict__ lists, int64_t n, int k, uint32_t* __restrict__ key,
                            uint32_t* __restrict__ val) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= n * k) return;
-  const int64_t i = t / k;
   const uint32_t v = label_id(lists[t].label);
-  const bool ok = v != kInvalidId && static_cast<int64_t>(v) < n && static_cast<int64_t>(v) != i;
+  const bool ok = v != kInvalidId && static_cast<int64_t>(v) < n;
   key[t] = ok ? v : static_cast<uint32_t>(n);
-  val[t] = static_cast<uint32_t>(i);
+  val[t] = static_cast<uint32_t>(t / k);
+}
+
+__global__ void k_seg_len(const int32_t* __restrict__ rs, const int32_t* __restrict__ re, int64_t n,
+                          int32_t* __restrict__ counts) {
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v < n) counts[v] = re[v] - rs[v];
 }
 
 __device__ __forceinline__ int64_t clique_share(int64_t v, const int32_t* ahrank, int64_t h) {
@@ -216,7 +224,7 @@ __global__ __launch_bounds__(256) void k_row_count(const hdb_nnd_element* __rest
     const uint32_t y = p < b1 ? rev[p] : 0xFFFFFFFFu;
     uint32_t py = __shfl(y, static_cast<int>((lane + 63) & 63));
     if (lane == 0) py = carry;
-    bool keep = p < b1 && y != py;
+    bool keep = p < b1 && y != py && static_cast<int64_t>(y) != v;
     // drop ids that are own entries (mutual kNN pairs are listed once)
     for (uint64_t m = bo; m; m &= m - 1) {
       const int l = __builtin_ctzll(m);
@@ -303,17 +311,43 @@ __global__ void k_ah_rank(const int32_t* __restrict__ ah, int64_t h, int32_t* __
   if (i < h) rank[ah[i]] = static_cast<int32_t>(i);
 }
 
-int mrd_graph(hdb_ctx* ctx, const float* X, const hdb_nnd_element* lists, const float* core, const int32_t* ah,
-              int64_t h, int64_t n, int d, int k, int64_t* nindex, int32_t* nlist, float* w, int64_t* edges_out) {
-  if (!X || !lists || !core || !nindex || !edges_out || n < 1 || d < 1 || k < 1 || h < 0 || (h > 0 && !ah))
-    return ctx->fail(HDB_EINVAL, "bad mrd_graph arguments");
+// Reverse pairs of the lists, sorted by row, with per-row segment bounds
+// (kept in the context's scratch for mrd_rows); counts (optional) receives
+// the in-degrees.
+int mrd_prep(hdb_ctx* ctx, const hdb_nnd_element* lists, int64_t n, int k, int32_t* counts) {
+  if (!lists || n < 1 || k < 1) return ctx->fail(HDB_EINVAL, "bad mrd_graph arguments");
   if (k > 64) return ctx->fail(HDB_EUNSUPPORTED, "lists longer than 64 entries");
   const int64_t m = n * k;
   if (m >= (1ll << 31) || n >= (1ll << 31) - 1) return ctx->fail(HDB_EUNSUPPORTED, "more than 2^31 list entries");
-  Stage st(ctx, "mrd_graph");
+  Stage st(ctx, "mrd_prep");
   uint32_t *key, *rev;
-  int32_t *own_sorted, *cnt, *ahrank;
+  int32_t* cnt;
   HDB_CALL(scratch_t(ctx, kSlotRowCursor, m, &key));
+  HDB_CALL(scratch_t(ctx, kSlotRowTmp, m, &rev));
+  HDB_CALL(scratch_t(ctx, kSlotRowCnt, 5 * n, &cnt));
+  int32_t* rs = cnt + 3 * n;
+  int32_t* re = cnt + 4 * n;
+  hipLaunchKernelGGL(k_rev_emit, dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream, lists, n, k, key, rev);
+  HDB_LAUNCHED();
+  HDB_CALL(radix_sort_u32_u32(ctx, key, rev, m, 0, bits_for(n)));
+  HDB_CALL(segment_bounds(ctx, key, m, n, rs, re));
+  if (counts) {
+    hipLaunchKernelGGL(k_seg_len, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, rs, re, n, counts);
+    HDB_LAUNCHED();
+  }
+  return HDB_OK;
+}
+
+// Rows of the MRD CSR from the state mrd_prep left for the same lists.
+int mrd_rows(hdb_ctx* ctx, const float* X, const hdb_nnd_element* lists, const float* core, const int32_t* ah,
+             int64_t h, int64_t n, int d, int k, int64_t* nindex, int32_t* nlist, float* w, int64_t* edges_out) {
+  if (!X || !lists || !core || !nindex || !edges_out || n < 1 || d < 1 || k < 1 || k > 64 || h < 0 ||
+      (h > 0 && !ah))
+    return ctx->fail(HDB_EINVAL, "bad mrd_graph arguments");
+  const int64_t m = n * k;
+  Stage st(ctx, "mrd_rows");
+  uint32_t* rev;
+  int32_t *own_sorted, *cnt, *ahrank;
   HDB_CALL(scratch_t(ctx, kSlotRowTmp, m, &rev));
   HDB_CALL(scratch_t(ctx, kSlotRowUniq, m, &own_sorted));
   HDB_CALL(scratch_t(ctx, kSlotRowCnt, 5 * n, &cnt));
@@ -328,10 +362,6 @@ int mrd_graph(hdb_ctx* ctx, const float* X, const hdb_nnd_element* lists, const 
     hipLaunchKernelGGL(k_ah_rank, dim3(grid_for(h, 256)), dim3(256), 0, ctx->stream, ah, h, ahrank);
     HDB_LAUNCHED();
   }
-  hipLaunchKernelGGL(k_rev_emit, dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream, lists, n, k, key, rev);
-  HDB_LAUNCHED();
-  HDB_CALL(radix_sort_u32_u32(ctx, key, rev, m, 0, bits_for(n)));
-  HDB_CALL(segment_bounds(ctx, key, m, n, rs, re));
   hipLaunchKernelGGL(k_row_count, dim3(grid_for(n, 4)), dim3(256), 0, ctx->stream, lists, n, k, rs, re, rev, ahrank,
                      h, own_sorted, uown, urev, total);
   HDB_LAUNCHED();
@@ -343,6 +373,15 @@ int mrd_graph(hdb_ctx* ctx, const float* X, const hdb_nnd_element* lists, const 
                      urev, nindex, core, ahrank, ah, h, nlist, w);
   HDB_LAUNCHED();
   return HDB_OK;
+}
+
+int mrd_graph(hdb_ctx* ctx, const float* X, const hdb_nnd_element* lists, const float* core, const int32_t* ah,
+              int64_t h, int64_t n, int d, int k, int64_t* nindex, int32_t* nlist, float* w, int64_t* edges_out) {
+  if (!X || !core || !nindex || !edges_out || d < 1 || h < 0 || (h > 0 && !ah))
+    return ctx->fail(HDB_EINVAL, "bad mrd_graph arguments");
+  Stage st(ctx, "mrd_graph");
+  HDB_CALL(mrd_prep(ctx, lists, n, k, nullptr));
+  return mrd_rows(ctx, X, lists, core, ah, h, n, d, k, nindex, nlist, w, edges_out);
 }
 
 }  // namespace hdb

@@ -268,6 +268,14 @@ int segment_bounds(hdb_ctx* ctx, const uint32_t* sorted_keys, int64_t m, int64_t
 int bits_for(int64_t max_key);
 // Keys-only u64 radix sort over bits [bit0, bit1).
 int radix_sort_u64(hdb_ctx* ctx, uint64_t* keys, int64_t n, int bit0, int bit1);
+// Input in nseg (<= 64) segments: logical index i in segment g (seg_pre[g] <=
+// i < seg_pre[g+1], device array) is at g * seg_cap + i - seg_pre[g].  The
+// sorted result is left where *kout / *vout point (the input buffers or the
+// sort's scratch), so these calls make no final copy.
+int radix_sort_u32_u64_segs(hdb_ctx* ctx, uint32_t* keys, uint64_t* vals, int64_t n, const int64_t* seg_pre,
+                            int nseg, int64_t seg_cap, int bit0, int bit1, uint32_t** kout, uint64_t** vout);
+int radix_sort_u32_u32_out(hdb_ctx* ctx, uint32_t* keys, uint32_t* vals, int64_t n, int bit0, int bit1,
+                           uint32_t** kout, uint32_t** vout);
 int read_i64(hdb_ctx* ctx, const int64_t* p, int64_t* out);
 int read_i32(hdb_ctx* ctx, const int32_t* p, int32_t* out);
 
@@ -289,7 +297,13 @@ int antihubs(hdb_ctx* ctx, const hdb_nnd_element* lists, const int32_t* counts, 
 int core_distances(hdb_ctx* ctx, const float* X, const hdb_nnd_element* lists, int64_t n, int d,
                    int k, int mpts, float* core);
 // Full (symmetric) MRD CSR, rows in neighbour-id order.  nlist == nullptr:
-// size query (nindex and *edges_out only).
+// size query (nindex and *edges_out only).  mrd_graph = mrd_prep (reverse
+// pairs sorted by row, in-degrees into counts when non-null) + mrd_rows,
+// which reads the state mrd_prep left in the context's scratch.
+int mrd_prep(hdb_ctx* ctx, const hdb_nnd_element* lists, int64_t n, int k, int32_t* counts);
+int mrd_rows(hdb_ctx* ctx, const float* X, const hdb_nnd_element* lists, const float* core,
+             const int32_t* ah, int64_t h, int64_t n, int d, int k,
+             int64_t* nindex, int32_t* nlist, float* w, int64_t* edges_out);
 int mrd_graph(hdb_ctx* ctx, const float* X, const hdb_nnd_element* lists, const float* core,
               const int32_t* ah, int64_t h, int64_t n, int d, int k,
               int64_t* nindex, int32_t* nlist, float* w, int64_t* edges_out);

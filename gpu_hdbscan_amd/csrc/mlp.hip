@@ -312,20 +312,37 @@ __global__ void k_mlp_loss(const float* __restrict__ Z, int64_t B, const float* 
 }
 
 // Triples for step `step`: anchor uniform, positive = a uniform entry of the
-// anchor's kNN list, negative uniform (xorshift64* streams).
+// anchor's kNN list; the negative is, three times in four, the end of a
+// kNegWalk-step random walk over the kNN lists (a point of the anchor's own
+// region beyond its neighbours, so decorrelating it splits large clusters
+// into several buckets) and otherwise uniform (xorshift64* streams).
+constexpr int kNegWalk = 3;
+
 __global__ void k_mlp_batch(const hdb_nnd_element* __restrict__ lists, int64_t n, int64_t B, uint64_t seed,
                             int32_t* __restrict__ rows) {
   const int64_t b = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (b >= B) return;
-  const uint64_t h0 = xorshift64star(seed ^ (static_cast<uint64_t>(b) * 0x9E3779B97F4A7C15ull));
-  const uint64_t h1 = xorshift64star(h0);
-  const uint64_t h2 = xorshift64star(h1);
-  const int64_t a = static_cast<int64_t>(h0 % static_cast<uint64_t>(n));
-  uint32_t pid = label_id(lists[a * kK + static_cast<int>(h1 % kK)].label);
+  uint64_t h = xorshift64star(seed ^ (static_cast<uint64_t>(b) * 0x9E3779B97F4A7C15ull));
+  const int64_t a = static_cast<int64_t>(h % static_cast<uint64_t>(n));
+  h = xorshift64star(h);
+  uint32_t pid = label_id(lists[a * kK + static_cast<int>(h % kK)].label);
   if (pid == kInvalidId || static_cast<int64_t>(pid) >= n) pid = static_cast<uint32_t>(a);
+  h = xorshift64star(h);
+  int64_t neg;
+  if ((h & 3u) != 0) {
+    neg = a;
+    for (int s = 0; s < kNegWalk; ++s) {
+      h = xorshift64star(h);
+      const uint32_t nx = label_id(lists[neg * kK + static_cast<int>(h % kK)].label);
+      if (nx != kInvalidId && static_cast<int64_t>(nx) < n) neg = nx;
+    }
+  } else {
+    h = xorshift64star(h);
+    neg = static_cast<int64_t>(h % static_cast<uint64_t>(n));
+  }
   rows[b] = static_cast<int32_t>(a);
   rows[B + b] = static_cast<int32_t>(pid);
-  rows[2 * B + b] = static_cast<int32_t>(h2 % static_cast<uint64_t>(n));
+  rows[2 * B + b] = static_cast<int32_t>(neg);
 }
 
 __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,

@@ -38,63 +38,63 @@ __device__ __forceinline__ int find_compress(int v, int* __restrict__ parent) {
   }
 }
 
-// Wave-aggregated append to a list.
-__device__ __forceinline__ int64_t wave_append(bool want, unsigned long long* counter) {
-  const uint64_t bal = __ballot(want);
-  if (!bal) return -1;
-  const int leader = __builtin_ctzll(bal);
-  unsigned long long base = 0;
-  if (static_cast<int>(__lane_id()) == leader) base = atomicAdd(counter, static_cast<unsigned long long>(__popcll(bal)));
-  base = __shfl(base, leader);
-  return want ? static_cast<int64_t>(base) + __popcll(bal & lanemask_lt()) : -1;
-}
-
 // Lightest edge leaving the component of each live vertex (sixteen lanes
-// per row).  A vertex whose row has no such edge stays dead: components only
-// grow.  Offers the weight to the component (cw), counts vertices with an
-// edge in `active`.
+// per row; the grid strides over groups of sixteen vertices).  A vertex
+// whose row has no such edge stays dead: components only grow.  Offers the
+// weight to the component (cw); one atomic per block adds the block's count
+// of vertices with an edge to `active` (same-address atomics serialise, so
+// there are few of them).
 __global__ __launch_bounds__(256) void k_bv_best(const int64_t* __restrict__ nindex, const int32_t* __restrict__ nlist,
                                                  const float* __restrict__ w, int64_t n, const int32_t* __restrict__ comp,
                                                  uint32_t* __restrict__ best_w, uint64_t* __restrict__ best_p,
                                                  uint32_t* __restrict__ cw, unsigned long long* __restrict__ active) {
-  const int64_t v = static_cast<int64_t>(blockIdx.x) * 16 + (threadIdx.x >> 4);
+  __shared__ unsigned long long s_cnt;
+  if (threadIdx.x == 0) s_cnt = 0;
+  __syncthreads();
   const int sl = static_cast<int>(threadIdx.x & 15);
-  uint32_t bw = kNoEdge;
-  uint64_t bp = ~0ull;
-  int rv = 0;
-  const bool live = v < n && best_w[v] != kNoEdge;
-  if (live) {
-    rv = comp[v];
-    const int64_t e1 = nindex[v + 1];
-    for (int64_t j = nindex[v] + sl; j < e1; j += 16) {
-      const int32_t u = nlist[j];
-      if (u < 0 || static_cast<int64_t>(u) >= n || comp[u] == rv) continue;
-      const uint32_t wb = wkey(w[j]);
-      const uint64_t pr = v < u ? ((static_cast<uint64_t>(v) << 32) | static_cast<uint32_t>(u))
-                                : ((static_cast<uint64_t>(u) << 32) | static_cast<uint64_t>(v));
-      if (wb < bw || (wb == bw && pr < bp)) {
-        bw = wb;
-        bp = pr;
+  unsigned long long mine = 0;
+  for (int64_t g = blockIdx.x; g * 16 < n; g += gridDim.x) {
+    const int64_t v = g * 16 + (threadIdx.x >> 4);
+    uint32_t bw = kNoEdge;
+    uint64_t bp = ~0ull;
+    int rv = 0;
+    const bool live = v < n && best_w[v] != kNoEdge;
+    if (live) {
+      rv = comp[v];
+      const int64_t e1 = nindex[v + 1];
+      for (int64_t j = nindex[v] + sl; j < e1; j += 16) {
+        const int32_t u = nlist[j];
+        if (u < 0 || static_cast<int64_t>(u) >= n || comp[u] == rv) continue;
+        const uint32_t wb = wkey(w[j]);
+        const uint64_t pr = v < u ? ((static_cast<uint64_t>(v) << 32) | static_cast<uint32_t>(u))
+                                  : ((static_cast<uint64_t>(u) << 32) | static_cast<uint64_t>(v));
+        if (wb < bw || (wb == bw && pr < bp)) {
+          bw = wb;
+          bp = pr;
+        }
+      }
+    }
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) {
+      const uint32_t ow = __shfl_xor(bw, o);
+      const uint64_t op = shfl_xor_u64(bp, o);
+      if (ow < bw || (ow == bw && op < bp)) {
+        bw = ow;
+        bp = op;
+      }
+    }
+    if (live && sl == 0) {
+      best_w[v] = bw;
+      best_p[v] = bp;
+      if (bw != kNoEdge) {
+        atomicMin(&cw[rv], bw);
+        ++mine;
       }
     }
   }
-#pragma unroll
-  for (int o = 8; o > 0; o >>= 1) {
-    const uint32_t ow = __shfl_xor(bw, o);
-    const uint64_t op = shfl_xor_u64(bp, o);
-    if (ow < bw || (ow == bw && op < bp)) {
-      bw = ow;
-      bp = op;
-    }
-  }
-  const bool lead = live && sl == 0;
-  if (lead) {
-    best_w[v] = bw;
-    best_p[v] = bp;
-    if (bw != kNoEdge) atomicMin(&cw[rv], bw);
-  }
-  const uint64_t bal = __ballot(lead && bw != kNoEdge);
-  if (__lane_id() == 0 && bal) atomicAdd(active, static_cast<unsigned long long>(__popcll(bal)));
+  if (mine) atomicAdd(&s_cnt, mine);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_cnt) atomicAdd(active, s_cnt);
 }
 
 // Among the vertices that offered the component's weight, the smallest pair.
@@ -112,37 +112,85 @@ __global__ void k_bv_pair(int64_t n, const int32_t* __restrict__ comp, const uin
 // The vertex holding its component's lightest edge records it and hooks its
 // root to the other side's root; when two components chose the same edge the
 // smaller root records it and the larger hooks under it (kernel2/kernel3,
-// mst.cu:174-196).  Roots are read from comp and written to next only.
-__global__ void k_bv_hook(int64_t n, const int32_t* __restrict__ comp, const uint32_t* __restrict__ best_w,
-                          const uint64_t* __restrict__ best_p, const uint32_t* __restrict__ cw,
-                          const unsigned long long* __restrict__ cp, int32_t* __restrict__ next,
-                          hdb_mst_edge* __restrict__ out, int64_t cap, unsigned long long* __restrict__ out_cnt) {
-  const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  bool rec = false;
-  uint32_t bw = kNoEdge;
-  uint64_t pr = 0;
-  if (v < n) {
-    bw = best_w[v];
-    if (bw != kNoEdge) {
-      const int rv = comp[v];
-      pr = best_p[v];
-      if (bw == cw[rv] && pr == cp[rv]) {
-        const int a = static_cast<int>(pr >> 32), b = static_cast<int>(static_cast<uint32_t>(pr));
-        const int u = a == static_cast<int>(v) ? b : a;
-        const int ru = comp[u];
-        const bool mutual = cw[ru] == bw && cp[ru] == pr;
-        rec = !mutual || rv < ru;
-        if (!mutual || rv > ru) next[rv] = ru;
-      }
+// mst.cu:174-196).  Roots are read from comp and written to next only.  The
+// grid strides over the vertices twice: once to count the block's records
+// and hook, once to write the records at the block's reserved offset.
+__device__ __forceinline__ bool bv_owner(int64_t v, const int32_t* __restrict__ comp,
+                                         const uint32_t* __restrict__ best_w, const uint64_t* __restrict__ best_p,
+                                         const uint32_t* __restrict__ cw, const unsigned long long* __restrict__ cp,
+                                         bool* rec, int* rv_out, int* ru_out, bool* mutual_out) {
+  const uint32_t bw = best_w[v];
+  if (bw == kNoEdge) return false;
+  const int rv = comp[v];
+  const uint64_t pr = best_p[v];
+  if (bw != cw[rv] || pr != cp[rv]) return false;
+  const int a = static_cast<int>(pr >> 32), b = static_cast<int>(static_cast<uint32_t>(pr));
+  const int u = a == static_cast<int>(v) ? b : a;
+  const int ru = comp[u];
+  const bool mutual = cw[ru] == bw && cp[ru] == pr;
+  *rec = !mutual || rv < ru;
+  *rv_out = rv;
+  *ru_out = ru;
+  *mutual_out = mutual;
+  return true;
+}
+
+__global__ __launch_bounds__(256) void k_bv_hook(int64_t n, const int32_t* __restrict__ comp,
+                                                 const uint32_t* __restrict__ best_w,
+                                                 const uint64_t* __restrict__ best_p, const uint32_t* __restrict__ cw,
+                                                 const unsigned long long* __restrict__ cp, int32_t* __restrict__ next,
+                                                 hdb_mst_edge* __restrict__ out, int64_t cap,
+                                                 unsigned long long* __restrict__ out_cnt) {
+  __shared__ unsigned long long s_cnt, s_base;
+  __shared__ int32_t s_wave[4];
+  if (threadIdx.x == 0) s_cnt = 0;
+  __syncthreads();
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  unsigned long long mine = 0;
+  for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n; v += stride) {
+    bool rec = false, mutual = false;
+    int rv = 0, ru = 0;
+    if (bv_owner(v, comp, best_w, best_p, cw, cp, &rec, &rv, &ru, &mutual)) {
+      if (!mutual || rv > ru) next[rv] = ru;
+      mine += rec ? 1 : 0;
     }
   }
-  const int64_t slot = wave_append(rec, out_cnt);
-  if (rec && slot < cap) {
-    hdb_mst_edge e;
-    e.from_node = static_cast<int32_t>(pr >> 32);
-    e.to_node = static_cast<int32_t>(static_cast<uint32_t>(pr));
-    e.weight = wval(bw);
-    out[slot] = e;
+  if (mine) atomicAdd(&s_cnt, mine);
+  __syncthreads();
+  if (threadIdx.x == 0) s_base = s_cnt ? atomicAdd(out_cnt, s_cnt) : 0ull;
+  __syncthreads();
+  // second sweep: same vertices in the same order; positions by block scan
+  int64_t base = static_cast<int64_t>(s_base);
+  const unsigned lane = __lane_id();
+  const unsigned wv = threadIdx.x >> 6;
+  for (int64_t v0 = static_cast<int64_t>(blockIdx.x) * blockDim.x; v0 < n; v0 += stride) {
+    const int64_t v = v0 + threadIdx.x;
+    bool rec = false, mutual = false;
+    int rv = 0, ru = 0;
+    const bool own = v < n && bv_owner(v, comp, best_w, best_p, cw, cp, &rec, &rv, &ru, &mutual);
+    rec = own && rec;
+    const uint64_t bal = __ballot(rec);
+    if (lane == 0) s_wave[wv] = __popcll(bal);
+    __syncthreads();
+    int64_t off = base;
+    int32_t tot = 0;
+    for (unsigned q = 0; q < 4; ++q) {
+      if (q < wv) off += s_wave[q];
+      tot += s_wave[q];
+    }
+    if (rec) {
+      const int64_t slot = off + __popcll(bal & lanemask_lt());
+      if (slot < cap) {
+        const uint64_t pr = best_p[v];
+        hdb_mst_edge e;
+        e.from_node = static_cast<int32_t>(pr >> 32);
+        e.to_node = static_cast<int32_t>(static_cast<uint32_t>(pr));
+        e.weight = wval(best_w[v]);
+        out[slot] = e;
+      }
+    }
+    base += tot;
+    __syncthreads();
   }
 }
 
@@ -157,11 +205,18 @@ __global__ void k_iota(int* __restrict__ p, int64_t n) {
   if (i < n) p[i] = static_cast<int>(i);
 }
 
-__global__ void k_count_roots(const int* __restrict__ comp, int64_t n, unsigned long long* __restrict__ cnt) {
-  const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const bool root = v < n && comp[v] == static_cast<int>(v);
-  const uint64_t bal = __ballot(root);
-  if (__lane_id() == 0 && bal) atomicAdd(cnt, static_cast<unsigned long long>(__popcll(bal)));
+__global__ __launch_bounds__(256) void k_count_roots(const int* __restrict__ comp, int64_t n,
+                                                     unsigned long long* __restrict__ cnt) {
+  __shared__ unsigned long long s_cnt;
+  if (threadIdx.x == 0) s_cnt = 0;
+  __syncthreads();
+  unsigned long long mine = 0;
+  for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n;
+       v += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    mine += comp[v] == static_cast<int>(v) ? 1 : 0;
+  if (mine) atomicAdd(&s_cnt, mine);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_cnt) atomicAdd(cnt, s_cnt);
 }
 
 // CSR order of the from < to entries: stable passes by `to`, then `from`.
@@ -211,6 +266,9 @@ int mst_csr(hdb_ctx* ctx, const int64_t* nindex, const int32_t* nlist, const flo
   HDB_CALL(scratch_t(ctx, kSlotMstMin, n, &cp));
   HDB_CALL(scratch_t(ctx, kSlotMstCnt, 4, &cnt));
   const unsigned gv = grid_for(n, 256);
+  // fixed grids for the kernels that end in a same-address atomic per block
+  const unsigned gs = static_cast<unsigned>(std::min<int64_t>(grid_for(n, 16), 2048));
+  const unsigned gh = static_cast<unsigned>(std::min<int64_t>(gv, 1024));
   hipLaunchKernelGGL(k_iota, dim3(gv), dim3(256), 0, ctx->stream, comp, n);
   HDB_LAUNCHED();
   hipLaunchKernelGGL(k_iota, dim3(gv), dim3(256), 0, ctx->stream, next, n);
@@ -223,15 +281,15 @@ int mst_csr(hdb_ctx* ctx, const int64_t* nindex, const int32_t* nlist, const flo
     HDB_CALL(fill_i32(ctx, reinterpret_cast<int32_t*>(cw), n, static_cast<int32_t>(kNoEdge)));
     HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(cp), n, ~0ull));
     HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(cnt), 1, 0));
-    hipLaunchKernelGGL(k_bv_best, dim3(grid_for(n, 16)), dim3(256), 0, ctx->stream, nindex, nlist, w, n, comp, best_w,
-                       best_p, cw, cnt);
+    hipLaunchKernelGGL(k_bv_best, dim3(gs), dim3(256), 0, ctx->stream, nindex, nlist, w, n, comp, best_w, best_p, cw,
+                       cnt);
     HDB_LAUNCHED();
     int64_t active = 0;
     HDB_CALL(read_i64(ctx, reinterpret_cast<const int64_t*>(cnt), &active));
     if (active == 0) break;
     hipLaunchKernelGGL(k_bv_pair, dim3(gv), dim3(256), 0, ctx->stream, n, comp, best_w, best_p, cw, cp);
     HDB_LAUNCHED();
-    hipLaunchKernelGGL(k_bv_hook, dim3(gv), dim3(256), 0, ctx->stream, n, comp, best_w, best_p, cw, cp, next, out, cap,
+    hipLaunchKernelGGL(k_bv_hook, dim3(gh), dim3(256), 0, ctx->stream, n, comp, best_w, best_p, cw, cp, next, out, cap,
                        cnt + 1);
     HDB_LAUNCHED();
     hipLaunchKernelGGL(k_bv_jump, dim3(gv), dim3(256), 0, ctx->stream, n, comp, next);
@@ -240,7 +298,7 @@ int mst_csr(hdb_ctx* ctx, const int64_t* nindex, const int32_t* nlist, const flo
   int64_t m = 0;
   HDB_CALL(read_i64(ctx, reinterpret_cast<const int64_t*>(cnt + 1), &m));
   HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(cnt), 1, 0));
-  hipLaunchKernelGGL(k_count_roots, dim3(gv), dim3(256), 0, ctx->stream, comp, n, cnt);
+  hipLaunchKernelGGL(k_count_roots, dim3(gh), dim3(256), 0, ctx->stream, comp, n, cnt);
   HDB_LAUNCHED();
   HDB_CALL(read_i64(ctx, reinterpret_cast<const int64_t*>(cnt), components_out));
   if (m > cap || m != n - *components_out) return ctx->fail(HDB_EDEVICE, "MST edge count does not match the forest");

@@ -34,6 +34,11 @@ namespace hdb {
 
 constexpr int kSMax = 16;  // sample list stride (SAMPLE_NUM <= 16)
 constexpr int kJoinWaves = 4;
+// Join blocks append proposals to one of kPropSegs segments (block % kPropSegs),
+// each with its own counter kCtrStride words apart: atomics on one address
+// serialise, so one counter for every block would limit the join.
+constexpr int kPropSegs = 64;
+constexpr int kCtrStride = 32;
 
 struct NndBufs {
   int32_t* fwd_new;
@@ -48,8 +53,9 @@ struct NndBufs {
   uint64_t* prop_key;
   int32_t* prop_s;  // n
   int32_t* prop_e;  // n
-  unsigned long long* prop_cnt;
+  unsigned long long* prop_cnt;  // kPropSegs counters, kCtrStride apart
   int64_t prop_cap;
+  int64_t seg_cap;  // proposals per segment: prop_cap / kPropSegs
   float* worst;
   unsigned long long* part;  // per join block: [0] proposals, [1] capacity drops, [2] reverse drops, [3] candidates
   int join_keep;
@@ -341,25 +347,30 @@ __device__ __forceinline__ void join_emit(const NndBufs& b, uint64_t (&top)[SLOT
     if (q < static_cast<int>(w)) woff += s_wcnt[q];
     total += s_wcnt[q];
   }
-  if (threadIdx.x == 0) *s_base = total ? atomicAdd(b.prop_cnt, static_cast<unsigned long long>(total)) : 0ull;
+  const int seg = static_cast<int>(blockIdx.x % kPropSegs);
+  if (threadIdx.x == 0)
+    *s_base = total ? atomicAdd(b.prop_cnt + seg * kCtrStride, static_cast<unsigned long long>(total)) : 0ull;
   __syncthreads();
   const int64_t base = static_cast<int64_t>(*s_base);
-  const bool fits = base + total <= b.prop_cap;
-  if (fits) {
-    int64_t o = base + woff + incl - cnt;
+  // entries past the segment's capacity are dropped one by one, so each
+  // segment holds exactly its first min(count, seg_cap) entries
+  int64_t o = base + woff + incl - cnt;
+  const int64_t seg0 = static_cast<int64_t>(seg) * b.seg_cap;
 #pragma unroll
-    for (int s = 0; s < SLOTS; ++s) {
-      if (top[s] != kKeyMax) {
-        b.prop_tgt[o] = my_id;
-        b.prop_key[o] = top[s];
-        ++o;
+  for (int s = 0; s < SLOTS; ++s) {
+    if (top[s] != kKeyMax) {
+      if (o < b.seg_cap) {
+        b.prop_tgt[seg0 + o] = my_id;
+        b.prop_key[seg0 + o] = top[s];
       }
+      ++o;
     }
   }
   if (threadIdx.x == 0) {
     unsigned long long* pp = b.part + 4 * static_cast<int64_t>(blockIdx.x);
     pp[0] += static_cast<unsigned long long>(total);
-    if (!fits) pp[1] += static_cast<unsigned long long>(total);
+    const int64_t over = base + total - b.seg_cap;
+    if (over > 0) pp[1] += static_cast<unsigned long long>(over < total ? over : total);
   }
   // reverse-sample drops (lanes 16 and 48 hold them) and candidate counts
   int rd = rev_drop;
@@ -777,21 +788,39 @@ __global__ void k_nnd_finalize(hdb_nnd_element* __restrict__ lists, int64_t tota
   }
 }
 
-__global__ void k_sum_parts(const unsigned long long* __restrict__ part, int64_t blocks,
-                            unsigned long long* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_sum_parts(const unsigned long long* __restrict__ part, int64_t blocks,
+                                                   unsigned long long* __restrict__ out) {
   __shared__ unsigned long long acc[4][256];
   unsigned long long a[4] = {0, 0, 0, 0};
-  for (int64_t i = threadIdx.x; i < blocks; i += 256)
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < blocks;
+       i += static_cast<int64_t>(gridDim.x) * 256)
     for (int q = 0; q < 4; ++q) a[q] += part[4 * i + q];
   for (int q = 0; q < 4; ++q) acc[q][threadIdx.x] = a[q];
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (static_cast<int>(threadIdx.x) < s)
-      for (int q = 0; q < 4; ++q) acc[q][threadIdx.x] += acc[q][threadIdx.x + s];
+  for (int st = 128; st > 0; st >>= 1) {
+    if (static_cast<int>(threadIdx.x) < st)
+      for (int q = 0; q < 4; ++q) acc[q][threadIdx.x] += acc[q][threadIdx.x + st];
     __syncthreads();
   }
-  if (threadIdx.x == 0)
-    for (int q = 0; q < 4; ++q) out[q] = acc[q][0];
+  if (threadIdx.x < 4 && acc[threadIdx.x][0]) atomicAdd(&out[threadIdx.x], acc[threadIdx.x][0]);
+}
+
+// Segment map of the proposal buffer: pre[g] = sum of min(count, seg_cap)
+// over the segments before g (one block of kPropSegs threads).
+__global__ void k_prop_segs(const unsigned long long* __restrict__ cnt, int64_t seg_cap, int64_t* __restrict__ pre) {
+  __shared__ int64_t c[kPropSegs];
+  const int g = static_cast<int>(threadIdx.x);
+  const int64_t v = static_cast<int64_t>(cnt[g * kCtrStride]);
+  c[g] = v < seg_cap ? v : seg_cap;
+  __syncthreads();
+  if (g == 0) {
+    int64_t acc = 0;
+    for (int q = 0; q < kPropSegs; ++q) {
+      pre[q] = acc;
+      acc += c[q];
+    }
+    pre[kPropSegs] = acc;
+  }
 }
 
 // --------------------------------------------------------- unique merge --
@@ -857,7 +886,11 @@ static int get_bufs(hdb_ctx* ctx, int64_t n, const hdb_params& prm, NndBufs* b) 
   HDB_CALL(scratch_t(ctx, kSlotRevSE, 4 * n, &rse));
   b->rev_s = rse;
   b->rev_e = rse + 2 * n;
-  b->prop_cap = n * static_cast<int64_t>(prm.bucket_cap);
+  // bucket_cap proposals per point on average, split into kPropSegs
+  // segments; a segment holds at least 4096 (a join block emits at most
+  // 4 waves x 64 lanes x 4 proposals = 1024)
+  b->seg_cap = std::max<int64_t>(n * static_cast<int64_t>(prm.bucket_cap) / kPropSegs, 4096);
+  b->prop_cap = b->seg_cap * kPropSegs;
   HDB_CALL(scratch_t(ctx, kSlotPropTgt, b->prop_cap, &b->prop_tgt));
   HDB_CALL(scratch_t(ctx, kSlotBucket, b->prop_cap, &b->prop_key));
   int32_t* pse;
@@ -865,13 +898,13 @@ static int get_bufs(hdb_ctx* ctx, int64_t n, const hdb_params& prm, NndBufs* b) 
   b->prop_s = pse;
   b->prop_e = pse + n;
   HDB_CALL(scratch_t(ctx, kSlotWorst, n, &b->worst));
-  unsigned long long* st;
-  HDB_CALL(scratch_t(ctx, kSlotStats, 8, &st));
+  unsigned long long* st;  // segment counters, then 4 totals, then the segment map
+  HDB_CALL(scratch_t(ctx, kSlotStats, kPropSegs * kCtrStride + 8 + kPropSegs + 1, &st));
   b->prop_cnt = st;
   const int64_t blocks = n;  // join grids use at most n blocks (one per point in BF16 mode)
   HDB_CALL(scratch_t(ctx, kSlotPart, 4 * blocks, &b->part));
   HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(b->part), 4 * blocks, 0));
-  HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(st), 8, 0));
+  HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(st), kPropSegs * kCtrStride + 8, 0));
   b->join_keep = prm.join_keep;
   b->distance_mode = prm.distance_mode;
   return HDB_OK;
@@ -879,10 +912,13 @@ static int get_bufs(hdb_ctx* ctx, int64_t n, const hdb_params& prm, NndBufs* b) 
 
 static int read_stats(hdb_ctx* ctx, int64_t n, const NndBufs& b) {
   const int64_t blocks = n;  // join grids use at most n blocks (one per point in BF16 mode)
-  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, ctx->stream, b.part, blocks, b.prop_cnt + 4);
+  unsigned long long* tot = b.prop_cnt + kPropSegs * kCtrStride;
+  HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(tot), 4, 0));
+  hipLaunchKernelGGL(k_sum_parts, dim3(static_cast<unsigned>(std::min<int64_t>(grid_for(blocks, 256), 512))),
+                     dim3(256), 0, ctx->stream, b.part, blocks, tot);
   HDB_LAUNCHED();
   unsigned long long s[4];
-  HDB_HIP(hipMemcpyAsync(s, b.prop_cnt + 4, sizeof(s), hipMemcpyDeviceToHost, ctx->stream));
+  HDB_HIP(hipMemcpyAsync(s, tot, sizeof(s), hipMemcpyDeviceToHost, ctx->stream));
   HDB_HIP(hipStreamSynchronize(ctx->stream));
   for (int i = 0; i < 4; ++i) ctx->stats_host[i] = static_cast<int64_t>(s[i]);
   return HDB_OK;
@@ -957,28 +993,36 @@ static int iterate(hdb_ctx* ctx, const float* X, int64_t n, int d, int S, bool m
                            split, b);
       HDB_LAUNCHED();
     }
+    // the sorts leave their results where they end (input or scratch
+    // buffers); the kernels that read them get those pointers
+    NndBufs bj = b;
     {
       Stage st(ctx, "nnd_reverse_sort");
-      HDB_CALL(radix_sort_u32_u32(ctx, b.rev_key, b.rev_val, n * 32, 0, rev_bits));
-      HDB_CALL(segment_bounds(ctx, b.rev_key, n * 32, 2 * n, b.rev_s, b.rev_e));
-      HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(b.prop_cnt), 1, 0));
+      uint32_t* rk = nullptr;
+      HDB_CALL(radix_sort_u32_u32_out(ctx, b.rev_key, b.rev_val, n * 32, 0, rev_bits, &rk, &bj.rev_val));
+      HDB_CALL(segment_bounds(ctx, rk, n * 32, 2 * n, b.rev_s, b.rev_e));
+      HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(b.prop_cnt), kPropSegs * kCtrStride, 0));
     }
     {
       Stage st(ctx, merge ? "nnd_join_merge" : "nnd_join");
-      if (merge) HDB_CALL(launch_join<false>(ctx, X, n, d, S, b));
-      else HDB_CALL(launch_join<true>(ctx, X, n, d, S, b));
+      if (merge) HDB_CALL(launch_join<false>(ctx, X, n, d, S, bj));
+      else HDB_CALL(launch_join<true>(ctx, X, n, d, S, bj));
     }
+    int64_t* seg_pre = reinterpret_cast<int64_t*>(b.prop_cnt + kPropSegs * kCtrStride + 8);
+    hipLaunchKernelGGL(k_prop_segs, dim3(1), dim3(kPropSegs), 0, ctx->stream, b.prop_cnt, b.seg_cap, seg_pre);
+    HDB_LAUNCHED();
     int64_t m = 0;
-    HDB_CALL(read_i64(ctx, reinterpret_cast<const int64_t*>(b.prop_cnt), &m));
-    m = std::min(m, b.prop_cap);
+    HDB_CALL(read_i64(ctx, seg_pre + kPropSegs, &m));
+    NndBufs bu = b;
     {
       Stage st(ctx, "nnd_proposal_sort");
-      HDB_CALL(radix_sort_u32_u64(ctx, b.prop_tgt, b.prop_key, m, 0, tgt_bits));
-      HDB_CALL(segment_bounds(ctx, b.prop_tgt, m, n, b.prop_s, b.prop_e));
+      HDB_CALL(radix_sort_u32_u64_segs(ctx, b.prop_tgt, b.prop_key, m, seg_pre, kPropSegs, b.seg_cap, 0, tgt_bits,
+                                       &bu.prop_tgt, &bu.prop_key));
+      HDB_CALL(segment_bounds(ctx, bu.prop_tgt, m, n, b.prop_s, b.prop_e));
     }
     {
       Stage st(ctx, merge ? "nnd_update_merge" : "nnd_update");
-      hipLaunchKernelGGL(k_nnd_update, dim3(grid_for(n, 4)), dim3(256), 0, ctx->stream, lists, n, b, it == 0 ? 1 : 0);
+      hipLaunchKernelGGL(k_nnd_update, dim3(grid_for(n, 4)), dim3(256), 0, ctx->stream, lists, n, bu, it == 0 ? 1 : 0);
       HDB_LAUNCHED();
     }
   }

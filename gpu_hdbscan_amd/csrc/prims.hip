@@ -151,17 +151,45 @@ constexpr int kRsWaveKeys = 64 * kRsChunks;
 constexpr int kRsTile = kRsBlock * kRsChunks;
 constexpr int kRsBins = 256;
 
+// Input read through a segment map: logical index i lives in segment g with
+// pre[g] <= i < pre[g+1], at physical index g * seg_cap + (i - pre[g]).
+// pre == nullptr: identity.  At most kRsMaxSegs segments.
+constexpr int kRsMaxSegs = 64;
+
+struct RsSegs {
+  const int64_t* pre;
+  int nseg;
+  int64_t seg_cap;
+};
+
+__device__ __forceinline__ void rs_load_segs(const RsSegs& sg, int64_t* s_pre) {
+  if (sg.pre)
+    for (int i = threadIdx.x; i <= sg.nseg; i += blockDim.x) s_pre[i] = sg.pre[i];
+}
+
+__device__ __forceinline__ int64_t rs_phys(const RsSegs& sg, const int64_t* s_pre, int64_t i) {
+  if (!sg.pre) return i;
+  int lo = 0, hi = sg.nseg;  // largest g with s_pre[g] <= i
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (s_pre[mid] <= i) lo = mid; else hi = mid;
+  }
+  return static_cast<int64_t>(lo) * sg.seg_cap + (i - s_pre[lo]);
+}
+
 template <typename K>
 __global__ void k_rs_hist(const K* __restrict__ keys, int64_t n, int shift, int64_t ntiles,
-                          int32_t* __restrict__ hist) {
+                          int32_t* __restrict__ hist, RsSegs sg) {
   __shared__ int32_t h[kRsBins];
+  __shared__ int64_t s_pre[kRsMaxSegs + 1];
   for (int i = threadIdx.x; i < kRsBins; i += kRsBlock) h[i] = 0;
+  rs_load_segs(sg, s_pre);
   __syncthreads();
   const int64_t base = static_cast<int64_t>(blockIdx.x) * kRsTile;
   for (int i = threadIdx.x; i < kRsTile; i += kRsBlock) {
     int64_t idx = base + i;
     if (idx < n) {
-      unsigned dgt = static_cast<unsigned>(keys[idx] >> shift) & 0xFFu;
+      unsigned dgt = static_cast<unsigned>(keys[rs_phys(sg, s_pre, idx)] >> shift) & 0xFFu;
       atomicAdd(&h[dgt], 1);
     }
   }
@@ -173,18 +201,23 @@ template <typename K, typename V, bool HAS_V>
 __global__ __launch_bounds__(kRsBlock) void k_rs_scatter(const K* __restrict__ kin, const V* __restrict__ vin,
                                                          K* __restrict__ kout, V* __restrict__ vout, int64_t n,
                                                          int shift, int64_t ntiles,
-                                                         const int64_t* __restrict__ hoff) {
+                                                         const int64_t* __restrict__ hoff, RsSegs sg) {
   static_assert(kRsBlock == kRsBins, "one thread per digit");
   __shared__ int32_t wcnt[kRsBlock / 64][kRsBins];
   __shared__ int64_t goff[kRsBins];
   __shared__ int32_t dstart[kRsBins];
   __shared__ int32_t wsum[kRsBlock / 64];
-  __shared__ K skey[kRsTile];
-  __shared__ V sval[HAS_V ? kRsTile : 1];
+  __shared__ int64_t s_pre[kRsMaxSegs + 1];
+  __shared__ uint8_t dig_of[HAS_V ? kRsTile : 1];
+  // staging: keys first, then values, through one buffer
+  __shared__ uint64_t s_stage[kRsTile * (sizeof(K) > sizeof(V) || !HAS_V ? sizeof(K) : sizeof(V)) / 8];
+  K* skey = reinterpret_cast<K*>(s_stage);
+  V* sval = reinterpret_cast<V*>(s_stage);
   const unsigned lane = __lane_id();
   const unsigned w = threadIdx.x >> 6;
   for (int i = threadIdx.x; i < (kRsBlock / 64) * kRsBins; i += kRsBlock) (&wcnt[0][0])[i] = 0;
   for (int i = threadIdx.x; i < kRsBins; i += kRsBlock) goff[i] = hoff[static_cast<int64_t>(i) * ntiles + blockIdx.x];
+  rs_load_segs(sg, s_pre);
   __syncthreads();
   const int64_t wbase = static_cast<int64_t>(blockIdx.x) * kRsTile + static_cast<int64_t>(w) * kRsWaveKeys;
   K key[kRsChunks];
@@ -196,8 +229,9 @@ __global__ __launch_bounds__(kRsBlock) void k_rs_scatter(const K* __restrict__ k
   for (int c = 0; c < kRsChunks; ++c) {
     const int64_t idx = wbase + c * 64 + lane;
     const bool act = idx < n;
-    key[c] = act ? kin[idx] : K(0);
-    if (HAS_V) val[c] = act ? vin[idx] : V(0);
+    const int64_t ph = act ? rs_phys(sg, s_pre, idx) : 0;
+    key[c] = act ? kin[ph] : K(0);
+    if (HAS_V) val[c] = act ? vin[ph] : V(0);
     const unsigned dg = static_cast<unsigned>(key[c] >> shift) & 0xFFu;
     dig[c] = static_cast<uint8_t>(dg);
     uint64_t m = __ballot(act);
@@ -240,16 +274,14 @@ __global__ __launch_bounds__(kRsBlock) void k_rs_scatter(const K* __restrict__ k
   for (unsigned ww = 0; ww < w; ++ww) wpre += wsum[ww];
   dstart[dgt] = wpre + incl - tot;
   __syncthreads();
-  // Stage the tile in LDS in digit order, then write each digit's run to
-  // consecutive global positions.
+  // Stage the tile in LDS in digit order (keys, then values, through one
+  // buffer) and write each digit's run to consecutive global positions.
+  int32_t lp[kRsChunks];
 #pragma unroll
   for (int c = 0; c < kRsChunks; ++c) {
     const int64_t idx = wbase + c * 64 + lane;
-    if (idx < n) {
-      const int32_t lp = dstart[dig[c]] + wcnt[w][dig[c]] + rank[c];
-      skey[lp] = key[c];
-      if (HAS_V) sval[lp] = val[c];
-    }
+    lp[c] = idx < n ? dstart[dig[c]] + wcnt[w][dig[c]] + rank[c] : -1;
+    if (lp[c] >= 0) skey[lp[c]] = key[c];
   }
   __syncthreads();
   const int64_t tbase = static_cast<int64_t>(blockIdx.x) * kRsTile;
@@ -257,15 +289,41 @@ __global__ __launch_bounds__(kRsBlock) void k_rs_scatter(const K* __restrict__ k
   for (int32_t i = static_cast<int32_t>(threadIdx.x); i < cnt; i += kRsBlock) {
     const K k = skey[i];
     const unsigned dd = static_cast<unsigned>(k >> shift) & 0xFFu;
-    const int64_t pos = goff[dd] + (i - dstart[dd]);
-    kout[pos] = k;
-    if (HAS_V) vout[pos] = sval[i];
+    kout[goff[dd] + (i - dstart[dd])] = k;
+    dig_of[i] = static_cast<uint8_t>(dd);
+  }
+  if (HAS_V) {
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < kRsChunks; ++c)
+      if (lp[c] >= 0) sval[lp[c]] = val[c];
+    __syncthreads();
+    for (int32_t i = static_cast<int32_t>(threadIdx.x); i < cnt; i += kRsBlock) {
+      const unsigned dd = dig_of[i];
+      vout[goff[dd] + (i - dstart[dd])] = sval[i];
+    }
   }
 }
 
+// LSD radix sort of (keys, vals) by bits [bit0, bit1).  seg != nullptr: the
+// input is read through that segment map (see RsSegs).  kout == nullptr: the
+// result ends in keys/vals; otherwise *kout/*vout receive the buffers that
+// hold it (keys/vals or the sort's own scratch), which saves a copy.
 template <typename K, typename V, bool HAS_V>
-static int radix_sort_t(hdb_ctx* ctx, K* keys, V* vals, int64_t n, int bit0, int bit1) {
-  if (n <= 1 || bit1 <= bit0) return HDB_OK;
+static int radix_sort_t(hdb_ctx* ctx, K* keys, V* vals, int64_t n, int bit0, int bit1, const RsSegs* seg,
+                        K** kout, V** vout) {
+  RsSegs sg{nullptr, 0, 0};
+  if (seg) {
+    if (seg->nseg < 1 || seg->nseg > kRsMaxSegs) return ctx->fail(HDB_EINVAL, "bad segment map");
+    sg = *seg;
+    if (bit1 < bit0 + 8) bit1 = bit0 + 8;  // at least one pass gathers the segments
+  }
+  if (n <= 1 && !seg) bit1 = bit0;
+  if (n <= 0 || bit1 <= bit0) {
+    if (kout) *kout = keys;
+    if (vout) *vout = vals;
+    return HDB_OK;
+  }
   const int64_t ntiles = (n + kRsTile - 1) / kRsTile;
   K* k2;
   V* v2 = nullptr;
@@ -281,14 +339,20 @@ static int radix_sort_t(hdb_ctx* ctx, K* keys, V* vals, int64_t n, int bit0, int
   V* vb = v2;
   int passes = 0;
   for (int shift = bit0; shift < bit1; shift += 8, ++passes) {
-    hipLaunchKernelGGL(k_rs_hist<K>, dim3(ntiles), dim3(kRsBlock), 0, ctx->stream, ka, n, shift, ntiles, hist);
+    const RsSegs sp = passes == 0 ? sg : RsSegs{nullptr, 0, 0};
+    hipLaunchKernelGGL(k_rs_hist<K>, dim3(ntiles), dim3(kRsBlock), 0, ctx->stream, ka, n, shift, ntiles, hist, sp);
     HDB_LAUNCHED();
     HDB_CALL(scan_i32_to_i64(ctx, hist, kRsBins * ntiles, hoff));
     hipLaunchKernelGGL((k_rs_scatter<K, V, HAS_V>), dim3(ntiles), dim3(kRsBlock), 0, ctx->stream, ka, va, kb, vb, n,
-                       shift, ntiles, hoff);
+                       shift, ntiles, hoff, sp);
     HDB_LAUNCHED();
     K* tk = ka; ka = kb; kb = tk;
     V* tv = va; va = vb; vb = tv;
+  }
+  if (kout) {
+    *kout = ka;
+    if (vout) *vout = va;
+    return HDB_OK;
   }
   if (passes & 1) {
     HDB_HIP(hipMemcpyAsync(keys, ka, static_cast<size_t>(n) * sizeof(K), hipMemcpyDeviceToDevice, ctx->stream));
@@ -298,13 +362,23 @@ static int radix_sort_t(hdb_ctx* ctx, K* keys, V* vals, int64_t n, int bit0, int
 }
 
 int radix_sort_u32_u64(hdb_ctx* ctx, uint32_t* keys, uint64_t* vals, int64_t n, int bit0, int bit1) {
-  return radix_sort_t<uint32_t, uint64_t, true>(ctx, keys, vals, n, bit0, bit1);
+  return radix_sort_t<uint32_t, uint64_t, true>(ctx, keys, vals, n, bit0, bit1, nullptr, nullptr, nullptr);
 }
 int radix_sort_u32_u32(hdb_ctx* ctx, uint32_t* keys, uint32_t* vals, int64_t n, int bit0, int bit1) {
-  return radix_sort_t<uint32_t, uint32_t, true>(ctx, keys, vals, n, bit0, bit1);
+  return radix_sort_t<uint32_t, uint32_t, true>(ctx, keys, vals, n, bit0, bit1, nullptr, nullptr, nullptr);
 }
 int radix_sort_u64(hdb_ctx* ctx, uint64_t* keys, int64_t n, int bit0, int bit1) {
-  return radix_sort_t<uint64_t, uint64_t, false>(ctx, keys, static_cast<uint64_t*>(nullptr), n, bit0, bit1);
+  return radix_sort_t<uint64_t, uint64_t, false>(ctx, keys, static_cast<uint64_t*>(nullptr), n, bit0, bit1, nullptr,
+                                                 nullptr, nullptr);
+}
+int radix_sort_u32_u64_segs(hdb_ctx* ctx, uint32_t* keys, uint64_t* vals, int64_t n, const int64_t* seg_pre,
+                            int nseg, int64_t seg_cap, int bit0, int bit1, uint32_t** kout, uint64_t** vout) {
+  const RsSegs sg{seg_pre, nseg, seg_cap};
+  return radix_sort_t<uint32_t, uint64_t, true>(ctx, keys, vals, n, bit0, bit1, &sg, kout, vout);
+}
+int radix_sort_u32_u32_out(hdb_ctx* ctx, uint32_t* keys, uint32_t* vals, int64_t n, int bit0, int bit1,
+                           uint32_t** kout, uint32_t** vout) {
+  return radix_sort_t<uint32_t, uint32_t, true>(ctx, keys, vals, n, bit0, bit1, nullptr, kout, vout);
 }
 
 }  // namespace hdb
