@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--no-learned", action="store_true", help="time the plain (random-init) form only")
     ap.add_argument("--learned-subsample", type=int, default=200000)
     ap.add_argument("--target", type=float, default=0.95, help="recall@32 a timed form must reach")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="after the run, three device-to-device copies of this size (counter calibration)")
     return ap.parse_args()
 
 
@@ -236,6 +238,12 @@ def main():
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(X, min(a.cpu_sample, n))
     print(json.dumps(out), flush=True)
+    if a.copy_calibration_mib > 0:
+        nb = a.copy_calibration_mib << 20
+        src, dst = ctx.empty((nb,), np.uint8), ctx.empty((nb,), np.uint8)
+        for _ in range(3):
+            ctx._check(ctx.lib.hdb_memcpy_d2d(ctx.handle, dst.ptr, src.ptr, nb))
+        ctx.synchronize()
     ctx.close()
 
 

@@ -89,10 +89,11 @@ PROTOTYPES = {
     "hdb_mlp_param_count": (C.c_int64, [VP]),
     "hdb_mlp_params": (C.c_int, [VP, VP, VP, VP]),
     "hdb_mlp_forward": (C.c_int, [VP, VP, VP, I64, I32, VP]),
-    "hdb_mlp_loss_grad": (C.c_int, [VP, VP, VP, I64, I32, VP, I64, C.POINTER(C.c_double), VP]),
-    "hdb_mlp_train": (C.c_int, [VP, VP, VP, I64, I32, VP, I32, I32, C.c_float, C.c_uint64,
+    "hdb_mlp_loss_grad": (C.c_int, [VP, VP, VP, I64, I32, VP, VP, I64, C.POINTER(C.c_double), VP]),
+    "hdb_mlp_train": (C.c_int, [VP, VP, VP, I64, I32, VP, VP, I32, I32, C.c_float, C.c_uint64,
                                 C.POINTER(C.c_double)]),
-    "hdb_learned_init": (C.c_int, [VP, VP, VP, I64, I32, C.c_uint64, VP]),
+    "hdb_rp_codes": (C.c_int, [VP, VP, I64, I32, C.c_uint64, VP]),
+    "hdb_learned_init": (C.c_int, [VP, C.POINTER(VP), I32, VP, I64, I32, C.c_uint64, VP]),
 }
 
 _lib = None

@@ -311,6 +311,82 @@ __global__ void k_mlp_loss(const float* __restrict__ Z, int64_t B, const float* 
   for (int i = kCode; i < kOutP; ++i) ga[i] = gp[i] = gn[i] = 0.0f;
 }
 
+// Loss of the partition-code target (rows [0, 3B), target s_b = +1 when bit
+// b of codes[rows[r]] is set, else -1) plus the kNN pair term over rows
+// (b, B + b):  L = sum_r sum_b (t_b - s_b)^2 / (12 * 3B)
+//                + sum_{b<B} |t_b - t_{B+b}|^2 / (12 * B);  t = tanh(z).
+// loss_part[r] is row r's share of L.
+__global__ void k_mlp_loss_bits(const float* __restrict__ Z, int64_t B, const int32_t* __restrict__ rows,
+                                const uint32_t* __restrict__ codes, float* __restrict__ dZ,
+                                float* __restrict__ loss_part) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= 3 * B) return;
+  const uint32_t c = codes[rows[r]];
+  const bool paired = r < 2 * B;
+  const int64_t other = r < B ? r + B : r - B;
+  const float inv_r = 1.0f / static_cast<float>(3 * B);
+  const float inv_b = 1.0f / static_cast<float>(B);
+  float lt = 0.0f, lp = 0.0f;
+  float* g = dZ + r * kOutP;
+  for (int i = 0; i < kCode; ++i) {
+    const float t = tanhf(Z[r * kOutP + i]);
+    const float e = t - (((c >> i) & 1u) ? 1.0f : -1.0f);
+    lt += e * e;
+    float dt = (2.0f / kCode) * e * inv_r;
+    if (paired) {
+      const float df = t - tanhf(Z[other * kOutP + i]);
+      lp += df * df;
+      dt += (2.0f / kCode) * df * inv_b;
+    }
+    g[i] = dt * (1.0f - t * t);
+  }
+  for (int i = kCode; i < kOutP; ++i) g[i] = 0.0f;
+  loss_part[r] = lt / kCode * inv_r + (r < B ? lp / kCode * inv_b : 0.0f);
+}
+
+// Balanced random-projection tree codes: at level l every cell (the l code
+// bits so far) splits at the median of x . r(l, cell); r is a Gaussian
+// direction drawn by Box-Muller from xorshift64*(seed, l, cell).  Keys are
+// (cell << 52) | (orderable projection bits << 20) | point id (n <= 2^20).
+__global__ void k_rp_keys(const float* __restrict__ X, int64_t n, int d, int level, uint64_t seed,
+                          const uint32_t* __restrict__ code, uint64_t* __restrict__ keys) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t c = code[i];
+  uint64_t h = xorshift64star(seed ^ (static_cast<uint64_t>(level + 1) << 44) ^
+                              (static_cast<uint64_t>(c) * 0x9E3779B97F4A7C15ull));
+  float proj = 0.0f;
+  for (int k = 0; k < d; k += 2) {
+    h = xorshift64star(h);
+    const float u1 = (static_cast<float>(h >> 40) + 1.0f) * (1.0f / 16777217.0f);
+    h = xorshift64star(h);
+    const float u2 = static_cast<float>(h >> 40) * (1.0f / 16777216.0f);
+    const float rad = sqrtf(-2.0f * logf(u1));
+    proj = __builtin_fmaf(X[i * d + k], rad * cosf(6.2831853f * u2), proj);
+    if (k + 1 < d) proj = __builtin_fmaf(X[i * d + k + 1], rad * sinf(6.2831853f * u2), proj);
+  }
+  uint32_t b = __float_as_uint(proj);
+  b = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+  keys[i] = (static_cast<uint64_t>(c) << 52) | (static_cast<uint64_t>(b) << 20) | static_cast<uint64_t>(i);
+}
+
+__global__ void k_rp_cells(const uint64_t* __restrict__ keys, int64_t n, uint32_t* __restrict__ cell) {
+  const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s < n) cell[s] = static_cast<uint32_t>(keys[s] >> 52);
+}
+
+// The lower half (floor) of each cell's sorted run gets bit 0, the rest 1.
+__global__ void k_rp_split(const uint64_t* __restrict__ keys, int64_t n, const int32_t* __restrict__ cs,
+                           const int32_t* __restrict__ ce, uint32_t* __restrict__ code) {
+  const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const uint64_t k = keys[s];
+  const uint32_t c = static_cast<uint32_t>(k >> 52);
+  const int64_t id = static_cast<int64_t>(k & 0xFFFFFull);
+  const int32_t half = (ce[c] - cs[c]) / 2;
+  code[id] = (c << 1) | ((s - cs[c]) >= half ? 1u : 0u);
+}
+
 // Triples for step `step`: anchor uniform, positive = a uniform entry of the
 // anchor's kNN list; the negative is, three times in four, the end of a
 // kNegWalk-step random walk over the kNN lists (a point of the anchor's own
@@ -549,7 +625,8 @@ int colsum(hdb_ctx* ctx, const float* G, int64_t R, int n, float* part, float* o
 }
 
 // One loss + gradient evaluation for the triples in `rows` (3B rows).
-int mlp_loss_grad(hdb_ctx* ctx, hdb_mlp* net, const float* X, int d, const int32_t* rows, int64_t B, double* loss) {
+int mlp_loss_grad(hdb_ctx* ctx, hdb_mlp* net, const float* X, int d, const int32_t* rows, int64_t B,
+                  const uint32_t* codes, double* loss) {
   const MlpLayout& L = net->L;
   const int64_t R = 3 * B;
   float *P1, *H1, *P2, *H2, *Z, *dZ, *dP2, *dP1, *part, *mean, *lossp;
@@ -559,16 +636,21 @@ int mlp_loss_grad(hdb_ctx* ctx, hdb_mlp* net, const float* X, int d, const int32
   H2 = P2 + R * kHid;
   HDB_CALL(scratch_t(ctx, kSlotMisc1, 2 * R * kHid, &dP2));
   dP1 = dP2 + R * kHid;
-  HDB_CALL(scratch_t(ctx, kSlotMisc2, 2 * R * kOutP + B + 64, &Z));
+  HDB_CALL(scratch_t(ctx, kSlotMisc2, 2 * R * kOutP + R + 64, &Z));
   dZ = Z + R * kOutP;
   lossp = dZ + R * kOutP;
-  mean = lossp + B;
+  mean = lossp + R;
   const int64_t splits = (R + 255) / 256;
   HDB_CALL(scratch_t(ctx, kSlotBfTmp, splits * std::max(L.kinp, kHid) * kHid, &part));
   HDB_CALL(mlp_forward(ctx, net, X, d, rows, R, P1, H1, P2, H2, Z));
-  hipLaunchKernelGGL(k_mlp_tmean, dim3(1), dim3(256), 0, ctx->stream, Z, R, mean);
-  HDB_LAUNCHED();
-  hipLaunchKernelGGL(k_mlp_loss, dim3(grid_for(B, 256)), dim3(256), 0, ctx->stream, Z, B, mean, dZ, lossp);
+  if (codes) {
+    hipLaunchKernelGGL(k_mlp_loss_bits, dim3(grid_for(R, 256)), dim3(256), 0, ctx->stream, Z, B, rows, codes, dZ,
+                       lossp);
+  } else {
+    hipLaunchKernelGGL(k_mlp_tmean, dim3(1), dim3(256), 0, ctx->stream, Z, R, mean);
+    HDB_LAUNCHED();
+    hipLaunchKernelGGL(k_mlp_loss, dim3(grid_for(B, 256)), dim3(256), 0, ctx->stream, Z, B, mean, dZ, lossp);
+  }
   HDB_LAUNCHED();
   const size_t blds = static_cast<size_t>(4) * (16 * 36 + 16 * (kHid + 4)) * sizeof(float);
   hipLaunchKernelGGL(k_mlp_bwd_rows, dim3(grid_for(R, 64)), dim3(256), blds, ctx->stream, R, net->prm, L, P1, P2, dZ,
@@ -581,7 +663,14 @@ int mlp_loss_grad(hdb_ctx* ctx, hdb_mlp* net, const float* X, int d, const int32
   HDB_CALL(colsum(ctx, dZ, R, kOutP, part, g + L.b3));
   HDB_CALL(colsum(ctx, dP2, R, kHid, part, g + L.b2));
   HDB_CALL(colsum(ctx, dP1, R, kHid, part, g + L.b1));
-  if (loss) {
+  if (loss && codes) {
+    std::vector<float> lp(R);
+    HDB_HIP(hipMemcpyAsync(lp.data(), lossp, R * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HDB_HIP(hipStreamSynchronize(ctx->stream));
+    double s = 0.0;
+    for (float x : lp) s += x;
+    *loss = s;
+  } else if (loss) {
     std::vector<float> lp(B);
     float mh[kCode];
     HDB_HIP(hipMemcpyAsync(lp.data(), lossp, B * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
@@ -679,11 +768,11 @@ int hdb_mlp_forward(hdb_ctx* ctx, hdb_mlp* net, const float* X, int64_t m, int32
 }
 
 int hdb_mlp_loss_grad(hdb_ctx* ctx, hdb_mlp* net, const float* X, int64_t n, int32_t d, const int32_t* rows,
-                      int64_t batch, double* loss, float* grad_host) {
+                      const uint32_t* codes, int64_t batch, double* loss, float* grad_host) {
   if (!ctx || !net || !X || !rows || batch < 1 || d != net->L.kin || n < 1)
     return ctx ? ctx->fail(HDB_EINVAL, "bad loss_grad args") : 1;
   HDB_HIP(hipSetDevice(ctx->device));
-  HDB_CALL(mlp_loss_grad(ctx, net, X, d, rows, batch, loss));
+  HDB_CALL(mlp_loss_grad(ctx, net, X, d, rows, batch, codes, loss));
   if (grad_host) {
     HDB_HIP(hipMemcpyAsync(grad_host, net->grad, net->L.total * 4, hipMemcpyDeviceToHost, ctx->stream));
     HDB_HIP(hipStreamSynchronize(ctx->stream));
@@ -692,7 +781,7 @@ int hdb_mlp_loss_grad(hdb_ctx* ctx, hdb_mlp* net, const float* X, int64_t n, int
 }
 
 int hdb_mlp_train(hdb_ctx* ctx, hdb_mlp* net, const float* X, int64_t n, int32_t d, const hdb_nnd_element* lists,
-                  int32_t steps, int32_t batch, float lr, uint64_t seed, double* final_loss) {
+                  const uint32_t* codes, int32_t steps, int32_t batch, float lr, uint64_t seed, double* final_loss) {
   if (!ctx || !net || !X || !lists || n < 2 || steps < 0 || batch < 1 || d != net->L.kin)
     return ctx ? ctx->fail(HDB_EINVAL, "bad train args") : 1;
   HDB_HIP(hipSetDevice(ctx->device));
@@ -703,7 +792,7 @@ int hdb_mlp_train(hdb_ctx* ctx, hdb_mlp* net, const float* X, int64_t n, int32_t
     hipLaunchKernelGGL(k_mlp_batch, dim3(grid_for(batch, 256)), dim3(256), 0, ctx->stream, lists, n,
                        static_cast<int64_t>(batch), seed + 0x100000001ull * static_cast<uint64_t>(net->step), rows);
     HDB_LAUNCHED();
-    HDB_CALL(mlp_loss_grad(ctx, net, X, d, rows, batch, (s == steps - 1) ? final_loss : nullptr));
+    HDB_CALL(mlp_loss_grad(ctx, net, X, d, rows, batch, codes, (s == steps - 1) ? final_loss : nullptr));
     net->step += 1;
     const float c1 = 1.0f - std::pow(0.9f, static_cast<float>(net->step));
     const float c2 = 1.0f - std::pow(0.999f, static_cast<float>(net->step));
@@ -714,55 +803,76 @@ int hdb_mlp_train(hdb_ctx* ctx, hdb_mlp* net, const float* X, int64_t n, int32_t
   return HDB_OK;
 }
 
-int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* net, const float* X, int64_t n, int32_t d, uint64_t seed,
-                     hdb_nnd_element* lists) {
-  if (!ctx || !net || !X || !lists || n < 34 || d != net->L.kin)
-    return ctx ? ctx->fail(HDB_EINVAL, "bad learned_init args (n must exceed 33)") : 1;
+int hdb_rp_codes(hdb_ctx* ctx, const float* X, int64_t n, int32_t d, uint64_t seed, uint32_t* codes) {
+  if (!ctx || !X || !codes || n < 2 || d < 1) return ctx ? ctx->fail(HDB_EINVAL, "bad rp_codes args") : 1;
+  if (n > (1ll << 20)) return ctx->fail(HDB_EUNSUPPORTED, "rp_codes takes at most 2^20 points");
+  HDB_HIP(hipSetDevice(ctx->device));
+  Stage st(ctx, "rp_codes");
+  uint64_t* keys;
+  uint32_t* cell;
+  int32_t* bounds;
+  HDB_CALL(scratch_t(ctx, kSlotSeedCand, n, &keys));
+  HDB_CALL(scratch_t(ctx, kSlotMisc1, n, &cell));
+  HDB_CALL(scratch_t(ctx, kSlotMisc2, 2 << kCode, &bounds));
+  HDB_CALL(fill_i32(ctx, reinterpret_cast<int32_t*>(codes), n, 0));
+  const unsigned g = grid_for(n, 256);
+  for (int l = 0; l < kCode; ++l) {
+    hipLaunchKernelGGL(k_rp_keys, dim3(g), dim3(256), 0, ctx->stream, X, n, d, l, seed, codes, keys);
+    HDB_LAUNCHED();
+    HDB_CALL(radix_sort_u64(ctx, keys, n, 20, 52 + l + 1));
+    hipLaunchKernelGGL(k_rp_cells, dim3(g), dim3(256), 0, ctx->stream, keys, n, cell);
+    HDB_LAUNCHED();
+    const int64_t ncell = 1ll << l;
+    HDB_CALL(segment_bounds(ctx, cell, n, ncell, bounds, bounds + ncell));
+    hipLaunchKernelGGL(k_rp_split, dim3(g), dim3(256), 0, ctx->stream, keys, n, bounds, bounds + ncell, codes);
+    HDB_LAUNCHED();
+  }
+  return HDB_OK;
+}
+
+int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* const* nets, int32_t tables, const float* X, int64_t n, int32_t d,
+                     uint64_t seed, hdb_nnd_element* lists) {
+  if (!ctx || !nets || tables < 1 || tables > kSeedTables || !X || !lists || n < 34)
+    return ctx ? ctx->fail(HDB_EINVAL, "bad learned_init args (1..4 networks, n must exceed 33)") : 1;
+  for (int t = 0; t < tables; ++t)
+    if (!nets[t] || nets[t]->L.kin != d) return ctx->fail(HDB_EINVAL, "network input width differs from d");
   if (n >= (1ll << 31)) return ctx->fail(HDB_EUNSUPPORTED, "more than 2^31 points");
   HDB_HIP(hipSetDevice(ctx->device));
   Stage st(ctx, "learned_init");
   float *Z, *dirs, *Xs;
   uint32_t *keys, *ids;
   uint64_t* cand;
-  const int64_t per_table = static_cast<int64_t>(kCode) * kCode + d;  // rotation + direction
   HDB_CALL(scratch_t(ctx, kSlotMisc0, n * kOutP, &Z));
   HDB_CALL(scratch_t(ctx, kSlotMisc1, n, &keys));
   HDB_CALL(scratch_t(ctx, kSlotMisc2, n, &ids));
-  HDB_CALL(scratch_t(ctx, kSlotBfTmp, kSeedTables * per_table, &dirs));
+  HDB_CALL(scratch_t(ctx, kSlotBfTmp, static_cast<int64_t>(kSeedTables) * d, &dirs));
   HDB_CALL(scratch_t(ctx, kSlotSeedX, n * d, &Xs));
   HDB_CALL(scratch_t(ctx, kSlotSeedCand, n * kSeedTables * kSeedKeep, &cand));
-  // per table: a random rotation of the code space (table 0: none) and a
-  // unit direction for the in-bucket order, Box-Muller from xorshift64*
-  std::vector<float> hd(static_cast<size_t>(kSeedTables * per_table));
+  // per table: a unit direction for the order inside a bucket, Box-Muller
+  // from xorshift64*
+  std::vector<float> hd(static_cast<size_t>(kSeedTables) * d);
   uint64_t s = seed * 0x2545F4914F6CDD1Dull + 7;
-  auto gauss = [&s]() {
-    s = xorshift64star(s);
-    const double u1 = ((s >> 11) + 1.0) / 9007199254740993.0;
-    s = xorshift64star(s);
-    const double u2 = (s >> 11) / 9007199254740992.0;
-    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
-  };
   for (int t = 0; t < kSeedTables; ++t) {
-    float* R = hd.data() + t * per_table;
-    for (int k = 0; k < kCode * kCode; ++k) R[k] = static_cast<float>(gauss());
-    float* u = R + kCode * kCode;
     double nrm = 0.0;
     for (int k = 0; k < d; ++k) {
-      u[k] = static_cast<float>(gauss());
-      nrm += static_cast<double>(u[k]) * u[k];
+      s = xorshift64star(s);
+      const double u1 = ((s >> 11) + 1.0) / 9007199254740993.0;
+      s = xorshift64star(s);
+      const double u2 = (s >> 11) / 9007199254740992.0;
+      hd[t * d + k] = static_cast<float>(std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2));
+      nrm += static_cast<double>(hd[t * d + k]) * hd[t * d + k];
     }
-    for (int k = 0; k < d; ++k) u[k] = static_cast<float>(u[k] / std::sqrt(nrm));
+    for (int k = 0; k < d; ++k) hd[t * d + k] = static_cast<float>(hd[t * d + k] / std::sqrt(nrm));
   }
   HDB_HIP(hipMemcpyAsync(dirs, hd.data(), hd.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-  HDB_CALL(mlp_forward(ctx, net, X, d, nullptr, n, nullptr, nullptr, nullptr, nullptr, Z));
+  if (tables < kSeedTables) HDB_CALL(fill_u64(ctx, cand, n * kSeedTables * kSeedKeep, kKeyMax));
   const size_t lds = static_cast<size_t>(kSeedRows) * (d + 1) * sizeof(float);
   const bool staged = lds <= 64 * 1024;
-  for (int t = 0; t < kSeedTables; ++t) {
-    const float* R = t == 0 ? nullptr : dirs + t * per_table;
-    const float* u = dirs + t * per_table + kCode * kCode;
+  for (int t = 0; t < tables; ++t) {
+    HDB_CALL(mlp_forward(ctx, nets[t], X, d, nullptr, n, nullptr, nullptr, nullptr, nullptr, Z));
     // x.u range: 64 covers the spread of a unit direction over typical data
-    hipLaunchKernelGGL(k_mlp_keys, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, Z, R, X, d, u, 64.0f, n, keys,
-                       ids);
+    hipLaunchKernelGGL(k_mlp_keys, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, Z, nullptr, X, d,
+                       dirs + t * d, 64.0f, n, keys, ids);
     HDB_LAUNCHED();
     HDB_CALL(radix_sort_u32_u32(ctx, keys, ids, n, 0, 32));
     hipLaunchKernelGGL(k_mlp_permute, dim3(grid_for(n * d, 256)), dim3(256), 0, ctx->stream, X, ids, n, d, Xs);

@@ -1,11 +1,13 @@
 """Learned initialiser for NN-Descent (no reference counterpart).
 
-LearnedInit.fit() runs plain NN-Descent on a seeded subsample, then trains
-the MLP on the GPU (hdb_mlp_train: BF16 MFMA forward/backward, Adam).
-nndescent_learned() seeds every list with the point's 32 neighbours in the
-(bucket code, x.u) order, runs NN-Descent from there, checks recall@32 on
-sampled queries against an exact GPU brute force, and falls back to plain
-random-init NN-Descent when the check fails.
+LearnedInit.fit() runs plain NN-Descent on a seeded subsample, builds one
+balanced random-projection-tree partition of it per table, and trains one
+MLP per table on the GPU (hdb_mlp_train: BF16 MFMA forward/backward, Adam)
+to reproduce its partition while keeping kNN pairs together.
+nndescent_learned() seeds every list with the nearest points around it in
+each table's (bucket code, x.u) order, runs NN-Descent from there, checks
+recall@32 on sampled queries against an exact GPU brute force, and falls
+back to plain random-init NN-Descent when the check fails.
 """
 from __future__ import annotations
 
@@ -20,19 +22,34 @@ from ._lib import NND_ELEMENT
 
 
 class LearnedInit:
-    def __init__(self, ctx: api.Context, d: int, seed: int = 0):
+    """`tables` networks (1..4), each trained to reproduce its own balanced
+    random-projection-tree partition of a subsample while keeping the
+    subsample's kNN pairs in one bucket; their sign codes are the bucket
+    tables of the initial lists."""
+
+    def __init__(self, ctx: api.Context, d: int, seed: int = 0, tables: int = 4):
+        if not 1 <= tables <= 4:
+            raise ValueError("tables must be in 1..4")
         self.ctx = ctx
         self.d = int(d)
-        h = C.c_void_p()
-        ctx._check(ctx.lib.hdb_mlp_create(ctx.handle, self.d, C.c_uint64(seed), C.byref(h)))
-        self.handle = h.value
+        self.seed = int(seed)
+        self.handles = []
+        for t in range(tables):
+            h = C.c_void_p()
+            ctx._check(ctx.lib.hdb_mlp_create(ctx.handle, self.d, C.c_uint64(seed * 16 + t), C.byref(h)))
+            self.handles.append(h.value)
         self.train_seconds = 0.0
         self.final_loss = float("nan")
 
+    @property
+    def handle(self):
+        return self.handles[0]
+
     def close(self):
-        if self.handle:
-            self.ctx.lib.hdb_mlp_destroy(self.handle)
-            self.handle = None
+        for h in self.handles:
+            if h:
+                self.ctx.lib.hdb_mlp_destroy(h)
+        self.handles = []
 
     def __del__(self):
         try:
@@ -44,54 +61,68 @@ class LearnedInit:
     def param_count(self) -> int:
         return int(self.ctx.lib.hdb_mlp_param_count(self.handle))
 
-    def get_params(self) -> np.ndarray:
+    def get_params(self, table: int = 0) -> np.ndarray:
         out = np.empty(self.param_count, np.float32)
-        self.ctx._check(self.ctx.lib.hdb_mlp_params(self.ctx.handle, self.handle, None, out.ctypes.data))
+        self.ctx._check(self.ctx.lib.hdb_mlp_params(self.ctx.handle, self.handles[table], None, out.ctypes.data))
         return out
 
-    def set_params(self, p) -> None:
+    def set_params(self, p, table: int = 0) -> None:
         p = np.ascontiguousarray(p, np.float32)
         assert p.size == self.param_count
-        self.ctx._check(self.ctx.lib.hdb_mlp_params(self.ctx.handle, self.handle, p.ctypes.data, None))
+        self.ctx._check(self.ctx.lib.hdb_mlp_params(self.ctx.handle, self.handles[table], p.ctypes.data, None))
 
-    def forward(self, X) -> np.ndarray:
+    def forward(self, X, table: int = 0) -> np.ndarray:
         Xd = self.ctx.dev(X, np.float32)
         Z = self.ctx.empty((Xd.shape[0], 16), np.float32)
-        self.ctx._check(self.ctx.lib.hdb_mlp_forward(self.ctx.handle, self.handle, Xd.ptr, Xd.shape[0], self.d, Z.ptr))
+        self.ctx._check(self.ctx.lib.hdb_mlp_forward(self.ctx.handle, self.handles[table], Xd.ptr, Xd.shape[0],
+                                                     self.d, Z.ptr))
         return Z.numpy()[:, :12]
 
-    def loss_grad(self, X, rows, batch):
+    def loss_grad(self, X, rows, batch, codes=None, table: int = 0):
+        """Loss and gradient on the given rows; codes (per point, or None for
+        the triple loss) select the partition-code loss."""
         Xd = self.ctx.dev(X, np.float32)
         r = self.ctx.dev(np.ascontiguousarray(rows, np.int32))
+        cd = None if codes is None else self.ctx.dev(np.ascontiguousarray(codes, np.uint32))
         loss = C.c_double()
         g = np.empty(self.param_count, np.float32)
-        self.ctx._check(self.ctx.lib.hdb_mlp_loss_grad(self.ctx.handle, self.handle, Xd.ptr, Xd.shape[0], self.d,
-                                                       r.ptr, int(batch), C.byref(loss), g.ctypes.data))
+        self.ctx._check(self.ctx.lib.hdb_mlp_loss_grad(self.ctx.handle, self.handles[table], Xd.ptr, Xd.shape[0],
+                                                       self.d, r.ptr, None if cd is None else cd.ptr, int(batch),
+                                                       C.byref(loss), g.ctypes.data))
         return loss.value, g
 
-    def train(self, Xsub, lists_sub, steps: int, batch: int = 4096, lr: float = 1e-3, seed: int = 0) -> float:
+    def train(self, Xsub, lists_sub, steps: int, batch: int = 4096, lr: float = 1e-3, seed: int = 0,
+              codes=None, table: int = 0) -> float:
         Xd = self.ctx.dev(Xsub, np.float32)
         Ld = lists_sub if isinstance(lists_sub, api.DeviceArray) else self.ctx.to_device(
             np.ascontiguousarray(lists_sub, NND_ELEMENT))
+        cd = None if codes is None else self.ctx.dev(codes, np.uint32)
         loss = C.c_double()
-        self.ctx._check(self.ctx.lib.hdb_mlp_train(self.ctx.handle, self.handle, Xd.ptr, Xd.shape[0], self.d, Ld.ptr,
-                                                   int(steps), int(batch), float(lr), C.c_uint64(seed),
-                                                   C.byref(loss)))
+        self.ctx._check(self.ctx.lib.hdb_mlp_train(self.ctx.handle, self.handles[table], Xd.ptr, Xd.shape[0],
+                                                   self.d, Ld.ptr, None if cd is None else cd.ptr, int(steps),
+                                                   int(batch), float(lr), C.c_uint64(seed), C.byref(loss)))
         self.final_loss = loss.value
         return loss.value
 
-    def fit(self, X: np.ndarray, n_sub: int = 200_000, nnd_iterations: int = 6, epochs: float = 2.0,
-            batch: int = 4096, lr: float = 2e-3, seed: int = 0) -> "LearnedInit":
-        """Plain NN-Descent on a seeded subsample, then GPU training."""
+    def fit(self, X: np.ndarray, n_sub: int = 200_000, nnd_iterations: int = 6, epochs: float = 8.0,
+            batch: int = 4096, lr: float = 2e-3, seed: int = 0, loss: str = "triple") -> "LearnedInit":
+        """Plain NN-Descent on a seeded subsample, then GPU training of each
+        table's network: loss "triple" (kNN positives, random-walk and uniform
+        negatives, bit balance; tables differ by their seeds) or "partition"
+        (each table fits its own balanced random-projection-tree partition)."""
+        if loss not in ("triple", "partition"):
+            raise ValueError("loss must be 'triple' or 'partition'")
         rng = np.random.default_rng(seed)
         n = X.shape[0]
-        sub = np.sort(rng.choice(n, size=min(n_sub, n), replace=False))
+        sub = np.sort(rng.choice(n, size=min(n_sub, n, 1 << 20), replace=False))
         Xs = np.ascontiguousarray(X[sub], np.float32)
         t0 = time.perf_counter()
         Xsd = self.ctx.to_device(Xs)
         Ls = api.NNDescent(self.ctx, Xsd, iteration=nnd_iterations, store_result_in_device=True)
         steps = max(1, int(epochs * len(sub) / batch))
-        self.train(Xsd, Ls, steps, batch, lr, seed)
+        for t in range(len(self.handles)):
+            codes = rp_codes(self.ctx, Xsd, seed=self.seed * 16 + t + 1) if loss == "partition" else None
+            self.train(Xsd, Ls, steps, batch, lr, seed + 97 * t, codes=codes, table=t)
         self.ctx.synchronize()
         self.train_seconds = time.perf_counter() - t0
         return self
@@ -100,9 +131,18 @@ class LearnedInit:
         Xd = self.ctx.dev(X, np.float32)
         n = Xd.shape[0]
         L = out if out is not None else self.ctx.empty((n, 32), NND_ELEMENT)
-        self.ctx._check(self.ctx.lib.hdb_learned_init(self.ctx.handle, self.handle, Xd.ptr, n, self.d,
+        arr = (C.c_void_p * len(self.handles))(*self.handles)
+        self.ctx._check(self.ctx.lib.hdb_learned_init(self.ctx.handle, arr, len(self.handles), Xd.ptr, n, self.d,
                                                       C.c_uint64(seed), L.ptr))
         return L
+
+
+def rp_codes(ctx: api.Context, X, seed: int = 0) -> api.DeviceArray:
+    """Balanced 12-level random-projection-tree cell of every point (device)."""
+    Xd = ctx.dev(X, np.float32)
+    codes = ctx.empty((Xd.shape[0],), np.uint32)
+    ctx._check(ctx.lib.hdb_rp_codes(ctx.handle, Xd.ptr, Xd.shape[0], Xd.shape[1], C.c_uint64(seed), codes.ptr))
+    return codes
 
 
 @dataclass

@@ -66,8 +66,8 @@ typedef struct {
   int32_t distance_mode;    /* 0: FP32 join; 1: BF16 MFMA join + FP32 re-rank */
   int32_t rerank_keep;      /* BF16 mode: survivors per candidate list (48) */
   int32_t bucket_cap;       /* proposal buffer capacity per iteration, in proposals
-                               per point (160); a workgroup whose proposals do not
-                               fit drops them and counts them */
+                               per point (160), split into 64 segments; proposals
+                               past a segment's capacity are dropped and counted */
   int32_t join_keep;        /* local join: 0 = reference pairing (new: nearest new +
                                nearest old partner; old: nearest new), T in 1..4 =
                                every candidate proposes its T nearest partners (4) */
@@ -225,17 +225,34 @@ int hdb_mlp_params(hdb_ctx* ctx, hdb_mlp* net, float* host_in, float* host_out);
 /* Z_dev: m x 16 (columns 12..15 are zero padding) */
 int hdb_mlp_forward(hdb_ctx* ctx, hdb_mlp* net, const float* X_dev, int64_t m, int32_t d, float* Z_dev);
 /* Loss and gradient for `batch` triples: rows_dev holds 3*batch point ids
- * (anchors, positives, negatives).  *loss (optional) and grad_host
- * (optional, param_count floats) receive the results. */
+ * (anchors, positives, rest).  codes_dev == NULL: the triple loss (anchor
+ * and positive pulled together, anchor and negative decorrelated, bits
+ * balanced); otherwise the partition-code loss, which fits the 12 output
+ * signs to codes_dev[id] on all 3*batch rows and pulls each (anchor,
+ * positive) pair together.  *loss (optional) and grad_host (optional,
+ * param_count floats) receive the results. */
 int hdb_mlp_loss_grad(hdb_ctx* ctx, hdb_mlp* net, const float* X_dev, int64_t n, int32_t d,
-                      const int32_t* rows_dev, int64_t batch, double* loss, float* grad_host);
-/* Adam steps on triples drawn from lists_dev (the kNN graph of X_dev). */
+                      const int32_t* rows_dev, const uint32_t* codes_dev, int64_t batch,
+                      double* loss, float* grad_host);
+/* Adam steps on triples drawn from lists_dev (the kNN graph of X_dev):
+ * anchors uniform, positives from the anchor's list, the rest mostly short
+ * random walks over the lists.  codes_dev selects the loss as above. */
 int hdb_mlp_train(hdb_ctx* ctx, hdb_mlp* net, const float* X_dev, int64_t n, int32_t d,
-                  const hdb_nnd_element* lists_dev, int32_t steps, int32_t batch, float lr,
-                  uint64_t seed, double* final_loss);
-/* Initial lists (ids only; distances are computed by hdb_nndescent_from). */
-int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* net, const float* X_dev, int64_t n, int32_t d,
-                     uint64_t seed, hdb_nnd_element* lists_dev);
+                  const hdb_nnd_element* lists_dev, const uint32_t* codes_dev, int32_t steps,
+                  int32_t batch, float lr, uint64_t seed, double* final_loss);
+/* Balanced 12-level random-projection tree over X_dev (n <= 2^20): at each
+ * level every cell splits at its median along a seeded Gaussian direction,
+ * giving 4096 cells of equal size (+-1); codes_dev receives each point's
+ * cell.  These are the training targets of the partition-code loss. */
+int hdb_rp_codes(hdb_ctx* ctx, const float* X_dev, int64_t n, int32_t d, uint64_t seed,
+                 uint32_t* codes_dev);
+/* Initial lists from 1..4 trained networks (ids with exact squared
+ * distances; hdb_nndescent_from recomputes them): network t's 12 sign bits
+ * are table t's bucket code, points are ordered by (code, x . u_t), and
+ * each point keeps the 16 nearest of the 128 points around it in each
+ * table; the 32 nearest distinct ids of all tables form its list. */
+int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* const* nets, int32_t tables, const float* X_dev,
+                     int64_t n, int32_t d, uint64_t seed, hdb_nnd_element* lists_dev);
 
 /* ---- host labels (HDBSCAN/trees/tree.cu) ----------------------------------- */
 /* Single linkage -> condensed tree (min cluster size mpts) -> stability ->

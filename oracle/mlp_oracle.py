@@ -8,8 +8,13 @@ the GPU kernels do when they form MFMA fragments; sums stay in float64.  The
 parameter vector follows the C ABI layout: W1[kinp][128], b1[128],
 W2[128][128], b2[128], W3[128][16], b3[16].
 
-Only tests/ may import this module.  The gradient is pinned against central
-finite differences of the loss (bf16=False).
+A second loss trains the network to reproduce a balanced partition code s
+(12 bits, s_b = +1 for bit b set, else -1) while keeping kNN pairs together:
+  L_bits = mean_rows sum_b (t_b - s_b)^2 / 12 + mean_b |t_a - t_p|^2 / 12
+over rows [0, 3B) (targets) and the pairs (row b, row B + b), b < B.
+
+Only tests/ may import this module.  Both gradients are pinned against
+central finite differences of their loss (bf16=False).
 """
 from __future__ import annotations
 
@@ -73,6 +78,55 @@ def forward(p, X, d_in, use_bf16=True):
     H2 = gelu(P2)
     Z = r(H2) @ r(W3) + b3
     return P1, H1, P2, H2, Z
+
+
+def _backward(p, d_in, Xr, P1, H1, P2, H2, t, dt, use_bf16):
+    W1, b1, W2, b2, W3, b3 = unpack(p, d_in)
+    r = bf16 if use_bf16 else (lambda a: np.asarray(a, np.float64))
+    dZ = dt * (1.0 - t * t)
+    dW3 = r(H2).T @ r(dZ)
+    db3 = dZ.sum(0)
+    dH2 = r(dZ) @ r(W3).T
+    dP2 = dH2 * gelu_grad(P2)
+    dW2 = r(H1).T @ r(dP2)
+    db2 = dP2.sum(0)
+    dH1 = r(dP2) @ r(W2).T
+    dP1 = dH1 * gelu_grad(P1)
+    dW1 = r(Xr).T @ r(dP1)
+    db1 = dP1.sum(0)
+    L = layout(d_in)
+    g = np.zeros(L["total"], np.float64)
+
+    def put(name, val, full_shape):
+        o, sz = L[name]
+        buf = np.zeros(full_shape)
+        buf[tuple(slice(0, n) for n in val.shape)] = val
+        g[o:o + sz] = buf.ravel()
+
+    put("w1", dW1, (L["kinp"], H))
+    put("b1", db1, (H,))
+    put("w2", dW2, (H, H))
+    put("b2", db2, (H,))
+    put("w3", dW3, (H, OUTP))
+    put("b3", db3, (OUTP,))
+    return g
+
+
+def loss_grad_bits(p, X, rows, B, codes, d_in, use_bf16=True):
+    """Loss L_bits and its gradient for rows[0:3B] of X with target codes
+    codes[rows] (pairs: rows[b], rows[B + b])."""
+    Xr = np.asarray(X, np.float32)[rows]
+    P1, H1, P2, H2, Z = forward(p, Xr, d_in, use_bf16)
+    t = np.tanh(Z)
+    R = 3 * B
+    c = np.asarray(codes, np.int64)[rows]
+    s = np.where((c[:, None] >> np.arange(CODE)) & 1, 1.0, -1.0)
+    ta, tp = t[:B], t[B:2 * B]
+    loss = ((t - s) ** 2).sum(1).mean() / CODE + ((ta - tp) ** 2).sum(1).mean() / CODE
+    dt = 2.0 * (t - s) / (CODE * R)
+    dt[:B] += 2.0 * (ta - tp) / (CODE * B)
+    dt[B:2 * B] -= 2.0 * (ta - tp) / (CODE * B)
+    return loss, _backward(p, d_in, Xr, P1, H1, P2, H2, t, dt, use_bf16)
 
 
 def loss_grad(p, X, rows, B, d_in, use_bf16=True):

@@ -39,6 +39,48 @@ def test_forward_and_gradient_match_fp64_oracle(ctx):
     assert np.linalg.norm(g - rg64) / np.linalg.norm(rg64) < 0.1
 
 
+def test_bits_loss_matches_fp64_oracle(ctx):
+    print("case: partition-code loss/gradient vs FP64 oracle", flush=True)
+    rng = np.random.default_rng(4)
+    d = 12
+    X = rng.standard_normal((1500, d)).astype(np.float32) * 3
+    net = learned.LearnedInit(ctx, d, seed=7, tables=1)
+    p = net.get_params()
+    codes = rng.integers(0, 4096, 1500).astype(np.uint32)
+    B = 400
+    rows = rng.integers(0, 1500, 3 * B).astype(np.int32)
+    loss, g = net.loss_grad(X, rows, B, codes=codes)
+    rloss, rg = M.loss_grad_bits(p.astype(np.float64), X, rows, B, codes, d, use_bf16=True)
+    print("bits loss", loss, rloss, flush=True)
+    assert abs(loss - rloss) <= 1e-3 * abs(rloss) + 1e-6
+    L = M.layout(d)
+    for name in ("w1", "b1", "w2", "b2", "w3", "b3"):
+        o, sz = L[name]
+        a, b = g[o:o + sz].astype(np.float64), rg[o:o + sz]
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
+        print(name, "rel err", rel, flush=True)
+        assert rel < 3e-2, name
+
+
+def test_rp_codes_balanced_and_local(ctx):
+    print("case: rp codes 30000x12", flush=True)
+    X = datasets.make("mix30_1m_d12", n=30000, seed=5)
+    c1 = learned.rp_codes(ctx, X, seed=3).numpy()
+    c2 = learned.rp_codes(ctx, X, seed=3).numpy()
+    np.testing.assert_array_equal(c1, c2)
+    cnt = np.bincount(c1, minlength=4096)
+    assert len(cnt) == 4096
+    # median splits at every level: leaf sizes differ by at most one
+    assert cnt.max() - cnt.min() <= 1, (cnt.min(), cnt.max())
+    L = api.NNDescent(ctx, X, iteration=6)
+    lab = L["label"][:, :8]
+    same_knn = (c1[lab] == c1[:, None]).mean()
+    rnd = np.random.default_rng(0).integers(0, len(X), lab.shape)
+    same_rnd = (c1[rnd] == c1[:, None]).mean()
+    print("kNN pairs sharing a cell", same_knn, "random pairs", same_rnd, flush=True)
+    assert same_knn > 20 * same_rnd
+
+
 def test_training_and_learned_init(ctx):
     print("case: learned init aniso200 60000x32", flush=True)
     X = datasets.make("aniso200_5m_d32", n=60000, seed=4)

@@ -47,3 +47,43 @@ def test_gradient_matches_finite_differences():
         pm[i] -= eps
         fd = (M.loss_grad(pp, X, rows, B, d, False)[0] - M.loss_grad(pm, X, rows, B, d, False)[0]) / (2 * eps)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (i, fd, g[i])
+
+
+def test_bits_gradient_matches_finite_differences():
+    rng = np.random.default_rng(1)
+    d = 12
+    L = M.layout(d)
+    p = np.zeros(L["total"], np.float64)
+    for name, scale in (("w1", 0.3), ("b1", 0.1), ("w2", 0.1), ("b2", 0.1), ("w3", 0.2), ("b3", 0.1)):
+        o, s = L[name]
+        p[o:o + s] = rng.standard_normal(s) * scale
+    o, s = L["w1"]
+    p[o:o + s].reshape(L["kinp"], M.H)[d:] = 0
+    o, s = L["w3"]
+    p[o:o + s].reshape(M.H, M.OUTP)[:, M.CODE:] = 0
+    o, s = L["b3"]
+    p[o + M.CODE:o + s] = 0
+    X = rng.standard_normal((40, d)).astype(np.float32)
+    codes = rng.integers(0, 4096, 40)
+    B = 6
+    rows = rng.integers(0, 40, 3 * B)
+    loss, g = M.loss_grad_bits(p, X, rows, B, codes, d, use_bf16=False)
+    # closed form at the loss level: t = tanh(z) recomputed independently
+    t = np.tanh(M.forward(p, X[rows], d, use_bf16=False)[4])
+    sgn = np.where((codes[rows][:, None] >> np.arange(12)) & 1, 1.0, -1.0)
+    ref = ((t - sgn) ** 2).mean() + ((t[:B] - t[B:2 * B]) ** 2).mean()
+    assert abs(loss - ref) < 1e-12
+    idx = []
+    for name in ("w1", "b1", "w2", "b2", "w3", "b3"):
+        o, s = L[name]
+        idx += list(rng.choice(np.arange(o, o + s), 6, replace=False))
+    eps = 1e-6
+    for i in idx:
+        if g[i] == 0.0 and abs(p[i]) == 0.0:
+            continue
+        pp, pm = p.copy(), p.copy()
+        pp[i] += eps
+        pm[i] -= eps
+        fd = (M.loss_grad_bits(pp, X, rows, B, codes, d, False)[0] -
+              M.loss_grad_bits(pm, X, rows, B, codes, d, False)[0]) / (2 * eps)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (i, fd, g[i])
