@@ -433,26 +433,15 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
   p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
 }
 
-// Sort keys of one bucket table: (12-bit code << 20) | 20-bit rank of x.u.
-// The code is the sign pattern of the 12-d embedding z, rotated by R
-// (12 x 12, row-major; null = identity, the trained bits); u is a fixed unit
-// direction and x.u is clamped to [-lim, lim].
-__global__ void k_mlp_keys(const float* __restrict__ Z, const float* __restrict__ R, const float* __restrict__ X,
-                           int d, const float* __restrict__ u, float lim, int64_t n, uint32_t* __restrict__ keys,
+// Sort keys of one bucket table: (12 sign bits of the embedding z << 20) |
+// 20-bit rank of x.u (u a fixed unit direction, x.u clamped to [-lim, lim]).
+__global__ void k_mlp_keys(const float* __restrict__ Z, const float* __restrict__ X, int d,
+                           const float* __restrict__ u, float lim, int64_t n, uint32_t* __restrict__ keys,
                            uint32_t* __restrict__ ids) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  float z[kCode];
-  for (int b = 0; b < kCode; ++b) z[b] = Z[i * kOutP + b];
   uint32_t code = 0;
-  for (int b = 0; b < kCode; ++b) {
-    float v = z[b];
-    if (R) {
-      v = 0.0f;
-      for (int c = 0; c < kCode; ++c) v = __builtin_fmaf(z[c], R[c * kCode + b], v);
-    }
-    code |= (v > 0.0f ? 1u : 0u) << b;
-  }
+  for (int b = 0; b < kCode; ++b) code |= (Z[i * kOutP + b] > 0.0f ? 1u : 0u) << b;
   float s = 0.0f;
   for (int k = 0; k < d; ++k) s = __builtin_fmaf(X[i * d + k], u[k], s);
   float t = (s / lim) * 0.5f + 0.5f;
@@ -871,8 +860,8 @@ int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* const* nets, int32_t tables, const f
   for (int t = 0; t < tables; ++t) {
     HDB_CALL(mlp_forward(ctx, nets[t], X, d, nullptr, n, nullptr, nullptr, nullptr, nullptr, Z));
     // x.u range: 64 covers the spread of a unit direction over typical data
-    hipLaunchKernelGGL(k_mlp_keys, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, Z, nullptr, X, d,
-                       dirs + t * d, 64.0f, n, keys, ids);
+    hipLaunchKernelGGL(k_mlp_keys, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, Z, X, d, dirs + t * d, 64.0f,
+                       n, keys, ids);
     HDB_LAUNCHED();
     HDB_CALL(radix_sort_u32_u32(ctx, keys, ids, n, 0, 32));
     hipLaunchKernelGGL(k_mlp_permute, dim3(grid_for(n * d, 256)), dim3(256), 0, ctx->stream, X, ids, n, d, Xs);

@@ -452,11 +452,10 @@ __global__ __launch_bounds__(kJoinWaves * 64) void k_nnd_join(const float* __res
 #pragma unroll
       for (int k = 0; k < kStride; ++k) vec[lane * kStride + k] = own[k];
     }
-  } else {
-    for (int c = 0; c < nc; ++c) {
-      const float* src = X + static_cast<int64_t>(s_cand[w][c]) * dd;
-      for (int k = lane; k < dd; k += 64) vec[c * stride + k] = src[k];
-    }
+  } else if (active) {
+    // lane c gathers candidate c's row: all rows are in flight together
+    const float* src = X + static_cast<int64_t>(my_id) * dd;
+    for (int k = 0; k < dd; ++k) vec[lane * stride + k] = src[k];
   }
   __syncthreads();
 
@@ -500,6 +499,12 @@ __global__ __launch_bounds__(kJoinWaves * 64) void k_nnd_join(const float* __res
   join_emit<WPB, kSlots>(b, top, active, my_id, nc, rev_drop, s_wcnt, s_new, &s_base);
 }
 
+// Order-preserving u32 image of a float of any sign (NaN excluded).
+__device__ __forceinline__ uint32_t order_bits(float x) {
+  const uint32_t b = __float_as_uint(x);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
 template <int SLOTS>
 __device__ __forceinline__ void top_insert32(uint32_t (&top)[SLOTS], uint32_t key) {
 #pragma unroll
@@ -510,15 +515,16 @@ __device__ __forceinline__ void top_insert32(uint32_t (&top)[SLOTS], uint32_t ke
   }
 }
 
-// Partner-split FP32 join for D <= 32.  Phase 1: every candidate computes its
-// squared distance to each new candidate (so each old-new pair is computed
-// once) and old candidates store theirs in an LDS block; phase 2: new
+// Partner-split FP32 join for D <= 32.  Phase 1: every candidate ranks each
+// new candidate by |a|^2 + |b|^2 - 2 a.b (so each old-new pair is evaluated
+// once) and old candidates store that value in an LDS block; phase 2: new
 // candidates read their old partners from that block.  Ranking uses 32-bit
-// keys -- the distance bits with the low 6 bits replaced by the partner's
-// index -- and the KEEP winners' exact squared distances are recomputed from
-// the same rows in the same operation order, so proposals carry exactly the
-// distances of k_nnd_join (pairings differ only among partners whose
-// distances agree to within 2^-17 relative).
+// keys -- the order-preserving bits of that value with the low 6 bits
+// replaced by the partner's index -- and the KEEP winners' exact squared
+// distances are then recomputed from the same rows in the same operation
+// order as k_nnd_join, so proposals carry exactly its distances; only the
+// choice among partners whose distances agree to about 1e-4 relative can
+// differ.
 template <int D, bool NEWNEW, int KEEP>
 __global__ __launch_bounds__(kJoinWaves * 64) void k_nnd_join2(const float* __restrict__ X, int64_t n, int d, int S,
                                                                NndBufs b) {
@@ -530,6 +536,7 @@ __global__ __launch_bounds__(kJoinWaves * 64) void k_nnd_join2(const float* __re
   constexpr uint32_t kNone = 0xFFFFFFFFu;
   __shared__ float s_vec[WPB][64 * kStride];
   __shared__ float s_blk[WPB][kBlock];
+  __shared__ float s_norm[WPB][64];
   __shared__ uint32_t s_new[WPB][64];
   __shared__ uint32_t s_old[WPB][64];
   __shared__ uint32_t s_cand[WPB][64];
@@ -557,10 +564,14 @@ __global__ __launch_bounds__(kJoinWaves * 64) void k_nnd_join2(const float* __re
   }
 #pragma unroll
   for (int k = D; k < kStride; ++k) own[k] = 0.0f;
+  float own_norm = 0.0f;
+#pragma unroll
+  for (int k = 0; k < D; ++k) own_norm = __builtin_fmaf(own[k], own[k], own_norm);
   if (active) {
 #pragma unroll
     for (int k = 0; k < kStride; ++k) s_vec[w][lane * kStride + k] = own[k];
   }
+  s_norm[w][lane] = own_norm;
   __syncthreads();
 
   const int bst = no + 1;  // row stride of the new x old block
@@ -570,15 +581,13 @@ __global__ __launch_bounds__(kJoinWaves * 64) void k_nnd_join2(const float* __re
   // phase 1: partners among the new candidates
   for (int j = 0; j < nn; ++j) {
     const float* row = s_vec[w] + j * kStride;
-    float acc = 0.0f;
+    float dot = 0.0f;
 #pragma unroll
-    for (int k = 0; k < D; ++k) {
-      const float diff = own[k] - row[k];
-      acc = __builtin_fmaf(diff, diff, acc);
-    }
+    for (int k = 0; k < D; ++k) dot = __builtin_fmaf(own[k], row[k], dot);
+    const float acc = __builtin_fmaf(-2.0f, dot, own_norm + s_norm[w][j]);
     if (active && !is_new) s_blk[w][j * bst + (static_cast<int>(lane) - nn)] = acc;
     const bool ok = active && j != static_cast<int>(lane) && (NEWNEW || !is_new);
-    const uint32_t key = ok ? ((__float_as_uint(acc) & ~63u) | static_cast<uint32_t>(j)) : kNone;
+    const uint32_t key = ok ? ((order_bits(acc) & ~63u) | static_cast<uint32_t>(j)) : kNone;
     if (KEEP == 0) top[0] = key < top[0] ? key : top[0];
     else top_insert32<kSlots>(top, key);
   }
@@ -586,7 +595,7 @@ __global__ __launch_bounds__(kJoinWaves * 64) void k_nnd_join2(const float* __re
   // phase 2: new candidates take their old partners from the block
   for (int o = 0; o < no; ++o) {
     const float acc = is_new ? s_blk[w][lane * bst + o] : 0.0f;
-    const uint32_t key = is_new ? ((__float_as_uint(acc) & ~63u) | static_cast<uint32_t>(nn + o)) : kNone;
+    const uint32_t key = is_new ? ((order_bits(acc) & ~63u) | static_cast<uint32_t>(nn + o)) : kNone;
     if (KEEP == 0) top[1] = key < top[1] ? key : top[1];
     else top_insert32<kSlots>(top, key);
   }
@@ -643,11 +652,20 @@ __global__ __launch_bounds__(64) void k_nnd_join_mfma(const float* __restrict__ 
   int nn, nc, rev_drop;
   join_candidates<WPB>(b, p, p < n, S, s_new, s_old, s_cand, &nn, &nc, &rev_drop);
 
-  // stage the FP32 rows (zero rows past nc and zero columns past d)
-  for (int c = 0; c < 64; ++c) {
-    const bool row_ok = c < nc;
-    const float* src = row_ok ? X + static_cast<int64_t>(s_cand[0][c]) * d : X;
-    for (int k = lane; k < dp; k += 64) vec[c * rs + k] = (row_ok && k < d) ? src[k] : 0.0f;
+  // stage the FP32 rows, lane c loading candidate c's row so that all 64
+  // gathers are in flight together (zero rows past nc, zero columns past d)
+  {
+    float* dst = vec + static_cast<int64_t>(lane) * rs;
+    int k0 = 0;
+    if (static_cast<int>(lane) < nc) {
+      const float* src = X + static_cast<int64_t>(s_cand[0][lane]) * d;
+      if ((d & 3) == 0) {
+        for (; k0 < d; k0 += 4) *reinterpret_cast<float4*>(dst + k0) = *reinterpret_cast<const float4*>(src + k0);
+      } else {
+        for (; k0 < d; ++k0) dst[k0] = src[k0];
+      }
+    }
+    for (int k = k0; k < dp; ++k) dst[k] = 0.0f;
   }
   __syncthreads();
   {
@@ -725,58 +743,83 @@ __global__ __launch_bounds__(64) void k_nnd_join_mfma(const float* __restrict__ 
 }
 
 // ----------------------------------------------------------- list update --
-// One wave per list: the 32 list entries (all marked old) and the list's
-// proposals (new), 96 at a time, are sorted as 128 keys; equal (distance, id)
-// keys collapse to the first (the old entry wins, as InsertToGlobalGraph
-// rejects an equal element); the first 32 survivors carry over.
+// InsertToGlobalGraph (nndescent.cu:969-1028) for one list per wave, with no
+// locks: lanes 0..31 hold the list (ascending keys, old flag cleared); each
+// chunk of up to 64 proposals for this target is sorted across the wave and
+// made unique, its 32 smallest distinct keys are reversed into lanes 32..63
+// (no other proposal of the chunk can be among the 32 smallest of the union),
+// and a 64-wide bitonic merge with the list keeps the 32 smallest distinct
+// (distance, id) keys -- an existing entry wins over a proposal of the same
+// id, so it stays old.
+__device__ __forceinline__ uint64_t bitonic_merge64(uint64_t v) {
+  const unsigned lane = __lane_id();
+#pragma unroll
+  for (int j = 32; j > 0; j >>= 1) {
+    const uint64_t o = shfl_xor_u64(v, j);
+    const uint64_t mn = v < o ? v : o, mx = v < o ? o : v;
+    v = (lane & j) == 0 ? mn : mx;
+  }
+  return v;
+}
+
 __global__ __launch_bounds__(256) void k_nnd_update(hdb_nnd_element* __restrict__ lists, int64_t n, NndBufs b,
                                                     int full) {
   __shared__ uint64_t s_top[4][32];
+  __shared__ uint64_t s_prop[4][32];
   const unsigned lane = __lane_id();
   const unsigned w = threadIdx.x >> 6;
   const int64_t t = static_cast<int64_t>(blockIdx.x) * 4 + w;
   if (t >= n) return;  // whole wave exits together; no block barriers below
   const int64_t ps = b.prop_s[t], pe = b.prop_e[t];
   hdb_nnd_element* out = lists + t * kK;
-  uint64_t r0 = kKeyMax, r1;
+  uint64_t cur = kKeyMax;
   if (lane < 32) {
     const hdb_nnd_element e = out[lane];
     const uint32_t id = label_id(e.label);
-    r0 = (id == kInvalidId) ? kKeyMax : make_key(e.distance, id, 0u);
+    cur = (id == kInvalidId) ? kKeyMax : make_key(e.distance, id, 0u);
   }
   if (ps == pe && !full) {  // no proposals: mark old
-    if (lane < 32) out[lane] = key_to_elem(r0);
+    if (lane < 32) out[lane] = key_to_elem(cur);
     return;
   }
   // one pass at least: the first update also removes duplicate init entries
   const uint64_t lt = lanemask_lt();
   int64_t c0 = ps;
   do {
-    if (lane >= 32) r0 = (c0 + lane - 32 < pe) ? b.prop_key[c0 + lane - 32] : kKeyMax;
-    r1 = (c0 + 32 + lane < pe) ? b.prop_key[c0 + 32 + lane] : kKeyMax;
-    bitonic128(r0, r1);
-    const uint64_t p0 = shfl_u64(r0, (lane + 63) & 63);
-    const uint64_t q1 = shfl_u64(r1, (lane + 63) & 63);
-    const uint64_t last0 = shfl_u64(r0, 63);
-    const uint64_t p1 = lane == 0 ? last0 : q1;
-    const bool keep0 = r0 != kKeyMax && (lane == 0 || (p0 >> 1) != (r0 >> 1));
-    const bool keep1 = r1 != kKeyMax && ((p1 >> 1) != (r1 >> 1));
-    const uint64_t b0 = __ballot(keep0), b1 = __ballot(keep1);
-    const int pos0 = __popcll(b0 & lt);
-    const int pos1 = __popcll(b0) + __popcll(b1 & lt);
-    const int total = __popcll(b0) + __popcll(b1);
-    if (keep0 && pos0 < kK) s_top[w][pos0] = r0;
-    if (keep1 && pos1 < kK) s_top[w][pos1] = r1;
+    uint64_t pr = (c0 + lane < pe) ? b.prop_key[c0 + lane] : kKeyMax;
+    pr = bitonic_group<64>(pr);
+    // the 32 smallest distinct proposals, compacted through LDS
+    const uint64_t pp = shfl_u64(pr, static_cast<int>((lane + 63) & 63));
+    const bool kp = pr != kKeyMax && (lane == 0 || pp != pr);
+    const uint64_t bp = __ballot(kp);
+    const int ppos = __popcll(bp & lt);
+    const int ptot = __popcll(bp);
+    if (kp && ppos < 32) s_prop[w][ppos] = pr;
+    if (static_cast<int>(lane) >= ptot && lane < 32u) s_prop[w][lane] = kKeyMax;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    const uint64_t pv = s_prop[w][(63u - lane) & 31u];
+    uint64_t v = lane < 32 ? cur : pv;
+    __builtin_amdgcn_wave_barrier();
+    v = bitonic_merge64(v);
+    const uint64_t prev = shfl_u64(v, static_cast<int>((lane + 63) & 63));
+    const bool keep = v != kKeyMax && (lane == 0 || (prev >> 1) != (v >> 1));
+    const uint64_t bal = __ballot(keep);
+    const int pos = __popcll(bal & lt);
+    const int total = __popcll(bal);
+    if (keep && pos < kK) s_top[w][pos] = v;
     if (static_cast<int>(lane) >= total && lane < 32u) s_top[w][lane] = kKeyMax;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    r0 = lane < 32 ? s_top[w][lane] : kKeyMax;
+    const uint64_t tv = s_top[w][lane & 31u];
+    cur = lane < 32 ? tv : kKeyMax;
     __builtin_amdgcn_wave_barrier();
-    c0 += 96;
+    c0 += 64;
   } while (c0 < pe);
-  if (lane < 32) out[lane] = key_to_elem(r0);
-  if (lane == 31) b.worst[t] = (r0 == kKeyMax) ? __uint_as_float(0x7F800000u) : key_d2(r0);
+  if (lane < 32) out[lane] = key_to_elem(cur);
+  if (lane == 31) b.worst[t] = (cur == kKeyMax) ? __uint_as_float(0x7F800000u) : key_d2(cur);
 }
 
 // Strip old flags so exported labels are >= 0.
