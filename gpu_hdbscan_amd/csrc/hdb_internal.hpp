@@ -261,10 +261,10 @@ int scan_i32_to_i64(hdb_ctx* ctx, const int32_t* in, int64_t n, int64_t* out);
 // Stable LSD radix sort.  Keys u32 (bits [bit0, bit1)) with u64 payload.
 int radix_sort_u32_u64(hdb_ctx* ctx, uint32_t* keys, uint64_t* vals, int64_t n, int bit0, int bit1);
 int radix_sort_u32_u32(hdb_ctx* ctx, uint32_t* keys, uint32_t* vals, int64_t n, int bit0, int bit1);
-// seg_start/seg_end[k] = [first, last+1) position of key k in sorted keys (k < nseg);
-// untouched keys keep the fill value 0.
+// seg_start/seg_end[k] = [first, last+1) position of segment k = key >> shift
+// in keys sorted by that value (k < nseg); untouched segments keep 0.
 int segment_bounds(hdb_ctx* ctx, const uint32_t* sorted_keys, int64_t m, int64_t nseg, int32_t* seg_start,
-                   int32_t* seg_end);
+                   int32_t* seg_end, int shift = 0);
 int bits_for(int64_t max_key);
 // Keys-only u64 radix sort over bits [bit0, bit1).
 int radix_sort_u64(hdb_ctx* ctx, uint64_t* keys, int64_t n, int bit0, int bit1);

@@ -7,8 +7,9 @@
 //   fwd_*     n x 16 int32 forward samples (+ counts n).
 //   rev pairs n x 32 (key = 2q + old, value = p) for every forward sample
 //             q of p, radix-sorted by key; rev_se = [start, end) per key.
-//   proposals (target u32, key u64) compacted per workgroup, radix-sorted by
-//             target; prop_se = [start, end) per target.
+//   proposals (target u32, key u64) appended per workgroup into 64 segments,
+//             radix-sorted by target >> 4; prop_se = [start, end) per group
+//             of 16 lists.
 //   worst     n f32: d2 of entry 31 of each list (proposal prefilter).
 //
 // One iteration (gpuknn/nndescent.cu:2133-2166):
@@ -20,12 +21,14 @@
 //                  TiledNewOldNeighborsCompareKernel (:405-510, :1030-1326):
 //                  one wave64 per point; candidates propose their nearest
 //                  partners to their own lists
-//   radix sort     proposals grouped by target list
+//   radix sort     proposals grouped by list group (target >> 4)
 //   k_nnd_update   MarkAllToOld + InsertToGlobalGraph (:1398-1403, :969-1028):
-//                  one wave per list merges the list with its proposals in
-//                  chunks of 96 through a 128-key bitonic sort; no locks, and
-//                  the result is the first 32 distinct keys of the union
-//                  whatever the proposal order.
+//                  one 4-wave block per group of 16 lists buckets the group's
+//                  proposals by list in LDS; a wave merges each of its four
+//                  buckets into its list 64 proposals at a time (sort,
+//                  unique, bitonic merge); no locks, and the result is the
+//                  first 32 distinct keys of the union whatever the proposal
+//                  order.
 #include <algorithm>
 
 #include "hdb_internal.hpp"
@@ -51,8 +54,8 @@ struct NndBufs {
   int32_t* rev_e;  // 2n
   uint32_t* prop_tgt;
   uint64_t* prop_key;
-  int32_t* prop_s;  // n
-  int32_t* prop_e;  // n
+  int32_t* prop_s;  // per list group (16 lists)
+  int32_t* prop_e;
   unsigned long long* prop_cnt;  // kPropSegs counters, kCtrStride apart
   int64_t prop_cap;
   int64_t seg_cap;  // proposals per segment: prop_cap / kPropSegs
@@ -743,14 +746,11 @@ __global__ __launch_bounds__(64) void k_nnd_join_mfma(const float* __restrict__ 
 }
 
 // ----------------------------------------------------------- list update --
-// InsertToGlobalGraph (nndescent.cu:969-1028) for one list per wave, with no
-// locks: lanes 0..31 hold the list (ascending keys, old flag cleared); each
-// chunk of up to 64 proposals for this target is sorted across the wave and
-// made unique, its 32 smallest distinct keys are reversed into lanes 32..63
-// (no other proposal of the chunk can be among the 32 smallest of the union),
-// and a 64-wide bitonic merge with the list keeps the 32 smallest distinct
-// (distance, id) keys -- an existing entry wins over a proposal of the same
-// id, so it stays old.
+// InsertToGlobalGraph (nndescent.cu:969-1028) without locks.  Proposals are
+// sorted by target >> 4, so a block owns a group of 16 lists and their
+// proposals: it stages the proposals in LDS tiles, buckets them by target
+// (order inside a bucket does not matter), and each wave merges four
+// buckets into their lists.
 __device__ __forceinline__ uint64_t bitonic_merge64(uint64_t v) {
   const unsigned lane = __lane_id();
 #pragma unroll
@@ -762,64 +762,134 @@ __device__ __forceinline__ uint64_t bitonic_merge64(uint64_t v) {
   return v;
 }
 
+// Merges up to 64 proposals (one per lane, kKeyMax for none) into the list
+// held in lanes 0..31 of cur (ascending keys; old entries carry flag 0) and
+// returns it: the chunk is sorted across the wave and made unique, its 32
+// smallest distinct keys are reversed into lanes 32..63 (no other proposal of
+// the chunk can be among the 32 smallest of the union), and a 64-wide
+// bitonic merge with the list keeps the 32 smallest distinct (distance, id)
+// keys -- an existing entry wins over a proposal of the same id, so it stays
+// old.  s_prop and s_top are this wave's 32-entry LDS rows.
+__device__ __forceinline__ uint64_t merge_proposals(uint64_t cur, uint64_t pr, uint64_t* s_prop, uint64_t* s_top) {
+  const unsigned lane = __lane_id();
+  const uint64_t lt = lanemask_lt();
+  pr = bitonic_group<64>(pr);
+  const uint64_t pp = shfl_u64(pr, static_cast<int>((lane + 63) & 63));
+  const bool kp = pr != kKeyMax && (lane == 0 || pp != pr);
+  const uint64_t bp = __ballot(kp);
+  const int ppos = __popcll(bp & lt);
+  const int ptot = __popcll(bp);
+  if (kp && ppos < 32) s_prop[ppos] = pr;
+  if (static_cast<int>(lane) >= ptot && lane < 32u) s_prop[lane] = kKeyMax;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  const uint64_t pv = s_prop[(63u - lane) & 31u];
+  uint64_t v = lane < 32 ? cur : pv;
+  __builtin_amdgcn_wave_barrier();
+  v = bitonic_merge64(v);
+  const uint64_t prev = shfl_u64(v, static_cast<int>((lane + 63) & 63));
+  const bool keep = v != kKeyMax && (lane == 0 || (prev >> 1) != (v >> 1));
+  const uint64_t bal = __ballot(keep);
+  const int pos = __popcll(bal & lt);
+  const int total = __popcll(bal);
+  if (keep && pos < kK) s_top[pos] = v;
+  if (static_cast<int>(lane) >= total && lane < 32u) s_top[lane] = kKeyMax;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  const uint64_t tv = s_top[lane & 31u];
+  __builtin_amdgcn_wave_barrier();
+  return lane < 32 ? tv : kKeyMax;
+}
+
+constexpr int kGroupBits = 4;     // lists per update block: 16
+constexpr int kGroupTile = 1024;  // proposals staged per LDS tile
+
+// One block of 4 waves per group g of lists [16g, 16g + 16); prop_s/prop_e[g]
+// bound the group's proposals; each wave merges four lists.  full: merge
+// (and so deduplicate) every list even without proposals (first iteration);
+// otherwise a list without proposals is only marked old.
 __global__ __launch_bounds__(256) void k_nnd_update(hdb_nnd_element* __restrict__ lists, int64_t n, NndBufs b,
                                                     int full) {
-  __shared__ uint64_t s_top[4][32];
+  constexpr int G = 1 << kGroupBits;
+  __shared__ uint64_t s_cur[G][32];
+  __shared__ uint64_t s_key[kGroupTile];
   __shared__ uint64_t s_prop[4][32];
+  __shared__ uint64_t s_top[4][32];
+  __shared__ int32_t s_cnt[G], s_start[G], s_hit[G];
   const unsigned lane = __lane_id();
   const unsigned w = threadIdx.x >> 6;
-  const int64_t t = static_cast<int64_t>(blockIdx.x) * 4 + w;
-  if (t >= n) return;  // whole wave exits together; no block barriers below
-  const int64_t ps = b.prop_s[t], pe = b.prop_e[t];
-  hdb_nnd_element* out = lists + t * kK;
-  uint64_t cur = kKeyMax;
-  if (lane < 32) {
-    const hdb_nnd_element e = out[lane];
-    const uint32_t id = label_id(e.label);
-    cur = (id == kInvalidId) ? kKeyMax : make_key(e.distance, id, 0u);
+  const int64_t g = blockIdx.x;
+  const int64_t t0 = g << kGroupBits;
+  for (int idx = threadIdx.x; idx < G * 32; idx += 256) {
+    const int q = idx >> 5, j = idx & 31;
+    const int64_t t = t0 + q;
+    uint64_t key = kKeyMax;
+    if (t < n) {
+      const hdb_nnd_element e = lists[t * kK + j];
+      const uint32_t id = label_id(e.label);
+      key = (id == kInvalidId) ? kKeyMax : make_key(e.distance, id, 0u);
+    }
+    s_cur[q][j] = key;
   }
-  if (ps == pe && !full) {  // no proposals: mark old
-    if (lane < 32) out[lane] = key_to_elem(cur);
-    return;
+  if (threadIdx.x < G) s_hit[threadIdx.x] = 0;
+  const int64_t gs = b.prop_s[g], ge = b.prop_e[g];
+  __syncthreads();
+  for (int64_t base = gs; base < ge; base += kGroupTile) {
+    const int cnt = static_cast<int>(ge - base < kGroupTile ? ge - base : kGroupTile);
+    if (threadIdx.x < G) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    int rb[kGroupTile / 256], rr[kGroupTile / 256];
+    uint64_t rk[kGroupTile / 256];
+#pragma unroll
+    for (int u = 0; u < kGroupTile / 256; ++u) {
+      const int i = static_cast<int>(threadIdx.x) + u * 256;
+      rb[u] = -1;
+      rr[u] = 0;
+      rk[u] = kKeyMax;
+      if (i < cnt) {
+        rb[u] = static_cast<int>(b.prop_tgt[base + i] & (G - 1));
+        rk[u] = b.prop_key[base + i];
+        rr[u] = atomicAdd(&s_cnt[rb[u]], 1);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < G) {
+      int acc = 0;
+      for (unsigned c = 0; c < threadIdx.x; ++c) acc += s_cnt[c];
+      s_start[threadIdx.x] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kGroupTile / 256; ++u)
+      if (rb[u] >= 0) s_key[s_start[rb[u]] + rr[u]] = rk[u];
+    __syncthreads();
+    for (int q4 = 0; q4 < G / 4; ++q4) {
+      const int q = static_cast<int>(w) * (G / 4) + q4;
+      const int c = s_cnt[q];
+      if (c == 0) continue;  // uniform across the wave
+      uint64_t cur = s_cur[q][lane & 31u];
+      cur = lane < 32 ? cur : kKeyMax;
+      for (int c0 = 0; c0 < c; c0 += 64) {
+        const uint64_t pr = c0 + static_cast<int>(lane) < c ? s_key[s_start[q] + c0 + lane] : kKeyMax;
+        cur = merge_proposals(cur, pr, s_prop[w], s_top[w]);
+      }
+      if (lane < 32) s_cur[q][lane] = cur;
+      if (lane == 0) s_hit[q] = 1;
+    }
+    __syncthreads();
   }
-  // one pass at least: the first update also removes duplicate init entries
-  const uint64_t lt = lanemask_lt();
-  int64_t c0 = ps;
-  do {
-    uint64_t pr = (c0 + lane < pe) ? b.prop_key[c0 + lane] : kKeyMax;
-    pr = bitonic_group<64>(pr);
-    // the 32 smallest distinct proposals, compacted through LDS
-    const uint64_t pp = shfl_u64(pr, static_cast<int>((lane + 63) & 63));
-    const bool kp = pr != kKeyMax && (lane == 0 || pp != pr);
-    const uint64_t bp = __ballot(kp);
-    const int ppos = __popcll(bp & lt);
-    const int ptot = __popcll(bp);
-    if (kp && ppos < 32) s_prop[w][ppos] = pr;
-    if (static_cast<int>(lane) >= ptot && lane < 32u) s_prop[w][lane] = kKeyMax;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    const uint64_t pv = s_prop[w][(63u - lane) & 31u];
-    uint64_t v = lane < 32 ? cur : pv;
-    __builtin_amdgcn_wave_barrier();
-    v = bitonic_merge64(v);
-    const uint64_t prev = shfl_u64(v, static_cast<int>((lane + 63) & 63));
-    const bool keep = v != kKeyMax && (lane == 0 || (prev >> 1) != (v >> 1));
-    const uint64_t bal = __ballot(keep);
-    const int pos = __popcll(bal & lt);
-    const int total = __popcll(bal);
-    if (keep && pos < kK) s_top[w][pos] = v;
-    if (static_cast<int>(lane) >= total && lane < 32u) s_top[w][lane] = kKeyMax;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    const uint64_t tv = s_top[w][lane & 31u];
-    cur = lane < 32 ? tv : kKeyMax;
-    __builtin_amdgcn_wave_barrier();
-    c0 += 64;
-  } while (c0 < pe);
-  if (lane < 32) out[lane] = key_to_elem(cur);
-  if (lane == 31) b.worst[t] = (cur == kKeyMax) ? __uint_as_float(0x7F800000u) : key_d2(cur);
+  for (int q4 = 0; q4 < G / 4; ++q4) {
+    const int q = static_cast<int>(w) * (G / 4) + q4;
+    const int64_t t = t0 + q;
+    if (t >= n) break;  // uniform across the wave; later lists are past n too
+    uint64_t cur = s_cur[q][lane & 31u];
+    cur = lane < 32 ? cur : kKeyMax;
+    if (full && !s_hit[q]) cur = merge_proposals(cur, kKeyMax, s_prop[w], s_top[w]);
+    if (lane < 32) lists[t * kK + lane] = key_to_elem(cur);
+    if (lane == 31) b.worst[t] = (cur == kKeyMax) ? __uint_as_float(0x7F800000u) : key_d2(cur);
+  }
 }
 
 // Strip old flags so exported labels are >= 0.
@@ -1025,6 +1095,7 @@ static int iterate(hdb_ctx* ctx, const float* X, int64_t n, int d, int S, bool m
                    hdb_nnd_element* lists, NndBufs& b) {
   const int rev_bits = bits_for(2 * n);
   const int tgt_bits = bits_for(n - 1);
+  const int64_t ngroups = (n + (1 << kGroupBits) - 1) >> kGroupBits;
   for (int it = 0; it < iterations; ++it) {
     {
       Stage st(ctx, merge ? "nnd_sample_merge" : "nnd_sample");
@@ -1059,13 +1130,15 @@ static int iterate(hdb_ctx* ctx, const float* X, int64_t n, int d, int S, bool m
     NndBufs bu = b;
     {
       Stage st(ctx, "nnd_proposal_sort");
-      HDB_CALL(radix_sort_u32_u64_segs(ctx, b.prop_tgt, b.prop_key, m, seg_pre, kPropSegs, b.seg_cap, 0, tgt_bits,
-                                       &bu.prop_tgt, &bu.prop_key));
-      HDB_CALL(segment_bounds(ctx, bu.prop_tgt, m, n, b.prop_s, b.prop_e));
+      // by list group (target >> kGroupBits); the update buckets each group
+      HDB_CALL(radix_sort_u32_u64_segs(ctx, b.prop_tgt, b.prop_key, m, seg_pre, kPropSegs, b.seg_cap, kGroupBits,
+                                       std::max(tgt_bits, kGroupBits), &bu.prop_tgt, &bu.prop_key));
+      HDB_CALL(segment_bounds(ctx, bu.prop_tgt, m, ngroups, b.prop_s, b.prop_e, kGroupBits));
     }
     {
       Stage st(ctx, merge ? "nnd_update_merge" : "nnd_update");
-      hipLaunchKernelGGL(k_nnd_update, dim3(grid_for(n, 4)), dim3(256), 0, ctx->stream, lists, n, bu, it == 0 ? 1 : 0);
+      hipLaunchKernelGGL(k_nnd_update, dim3(static_cast<unsigned>(ngroups)), dim3(256), 0, ctx->stream, lists, n, bu,
+                         it == 0 ? 1 : 0);
       HDB_LAUNCHED();
     }
   }

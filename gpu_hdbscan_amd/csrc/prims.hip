@@ -386,21 +386,21 @@ int radix_sort_u32_u32_out(hdb_ctx* ctx, uint32_t* keys, uint32_t* vals, int64_t
 namespace hdb {
 
 // -------------------------------------------------------- segment bounds --
-__global__ void k_seg_bounds(const uint32_t* __restrict__ keys, int64_t m, int64_t nseg, int32_t* __restrict__ s,
-                             int32_t* __restrict__ e) {
+__global__ void k_seg_bounds(const uint32_t* __restrict__ keys, int64_t m, int64_t nseg, int shift,
+                             int32_t* __restrict__ s, int32_t* __restrict__ e) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= m) return;
-  const uint32_t k = keys[i];
+  const uint32_t k = keys[i] >> shift;
   if (static_cast<int64_t>(k) >= nseg) return;
-  if (i == 0 || keys[i - 1] != k) s[k] = static_cast<int32_t>(i);
-  if (i == m - 1 || keys[i + 1] != k) e[k] = static_cast<int32_t>(i + 1);
+  if (i == 0 || (keys[i - 1] >> shift) != k) s[k] = static_cast<int32_t>(i);
+  if (i == m - 1 || (keys[i + 1] >> shift) != k) e[k] = static_cast<int32_t>(i + 1);
 }
 
-int segment_bounds(hdb_ctx* ctx, const uint32_t* keys, int64_t m, int64_t nseg, int32_t* s, int32_t* e) {
+int segment_bounds(hdb_ctx* ctx, const uint32_t* keys, int64_t m, int64_t nseg, int32_t* s, int32_t* e, int shift) {
   HDB_CALL(fill_i32(ctx, s, nseg, 0));
   HDB_CALL(fill_i32(ctx, e, nseg, 0));
   if (m <= 0) return HDB_OK;
-  hipLaunchKernelGGL(k_seg_bounds, dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream, keys, m, nseg, s, e);
+  hipLaunchKernelGGL(k_seg_bounds, dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream, keys, m, nseg, shift, s, e);
   HDB_LAUNCHED();
   return HDB_OK;
 }
