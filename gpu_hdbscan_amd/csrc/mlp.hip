@@ -104,7 +104,7 @@ __global__ __launch_bounds__(kMlpWaves * 64) void k_mlp_fwd(const float* __restr
                                                  float* __restrict__ P1, float* __restrict__ H1,
                                                  float* __restrict__ P2, float* __restrict__ H2,
                                                  float* __restrict__ Z) {
-  extern __shared__ float lds[];
+  extern __shared__ __attribute__((aligned(16))) float lds[];
   const unsigned l = __lane_id();
   const unsigned w = threadIdx.x >> 6;
   const int ldi = L.kinp + 4, lda = kHid + 4;
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(256) void k_mlp_bwd_rows(int64_t R, const float* __
                                                       const float* __restrict__ P1, const float* __restrict__ P2,
                                                       const float* __restrict__ dZ, float* __restrict__ dP2,
                                                       float* __restrict__ dP1) {
-  extern __shared__ float lds[];
+  extern __shared__ __attribute__((aligned(16))) float lds[];
   const unsigned l = __lane_id();
   const unsigned w = threadIdx.x >> 6;
   const int lda = kHid + 4;
@@ -482,7 +482,7 @@ __device__ __forceinline__ int64_t seed_lo(int64_t r, int64_t n) {
 template <bool kStaged>
 __global__ __launch_bounds__(256) void k_mlp_seed(const float* __restrict__ Xs, const uint32_t* __restrict__ order,
                                                   int64_t n, int d, int table, uint64_t* __restrict__ cand) {
-  extern __shared__ float lds_rows[];
+  extern __shared__ __attribute__((aligned(16))) float lds_rows[];
   const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kSeedPts;
   const int64_t sb = seed_lo(r0, n);
   const float* rows = Xs + sb * d;

@@ -421,7 +421,7 @@ __global__ __launch_bounds__(kJoinWaves * 64) void k_nnd_join(const float* __res
   constexpr bool kOwnReg = (D > 0 && D <= 32);
   constexpr int kStride = kOwnReg ? ((D + 3) & ~3) : 0;
   constexpr int kSlots = KEEP == 0 ? 2 : KEEP;
-  extern __shared__ float lds_dyn[];
+  extern __shared__ __attribute__((aligned(16))) float lds_dyn[];
   __shared__ uint32_t s_new[WPB][64];
   __shared__ uint32_t s_old[WPB][64];
   __shared__ uint32_t s_cand[WPB][64];
@@ -431,7 +431,7 @@ __global__ __launch_bounds__(kJoinWaves * 64) void k_nnd_join(const float* __res
   const unsigned w = threadIdx.x >> 6;
   const int64_t p = static_cast<int64_t>(blockIdx.x) * WPB + w;
   const int dd = D > 0 ? D : d;
-  const int stride = kOwnReg ? kStride : dd + 1;
+  const int stride = kOwnReg ? kStride : ((dd + 3) & ~3) + 4;  // 16-byte aligned rows
   float* vec = lds_dyn + static_cast<int64_t>(w) * 64 * stride;
 
   int nn, nc, rev_drop;
@@ -488,7 +488,7 @@ __global__ __launch_bounds__(kJoinWaves * 64) void k_nnd_join(const float* __res
         acc = __builtin_fmaf(diff, diff, acc);
       }
     } else {
-      acc = d2_any(vec + lane * stride, row, dd);
+      acc = (dd & 3) == 0 ? d2_vec4(vec + lane * stride, row, dd) : d2_any(vec + lane * stride, row, dd);
     }
     const bool ok = active && j >= jb && j < je && j != static_cast<int>(lane);
     const uint64_t key = ok ? make_key(acc, s_cand[w][j], 1u) : kKeyMax;
@@ -624,22 +624,35 @@ __global__ __launch_bounds__(kJoinWaves * 64) void k_nnd_join2(const float* __re
 
 // BF16 join (distance_mode 1): the 64 x 64 Gram matrix of the candidates is
 // computed with v_mfma_f32_16x16x32_bf16 (BF16 inputs, FP32 accumulation)
-// from the FP32 rows staged in LDS; approximate squared distances
-// |x|^2 + |y|^2 - 2 x.y pick each candidate's SHORT nearest partners, which
-// are re-ranked with the exact FP32 formula, so proposals -- and therefore
-// lists, core distances and MST weights -- carry exact FP32 distances only.
+// from the FP32 rows staged in LDS, with the operands swapped so that lane l
+// holds row I*16 + (l % 16) and columns J*16 + 4*(l / 16) + q of tile (I, J).
+// Each lane ranks its rows' partners by the approximate |x|^2 + |y|^2 - 2 x.y
+// (32-bit keys), the four lanes that share a row merge their shortlists, and
+// each candidate's SHORT nearest partners are re-ranked with the exact FP32
+// formula, so proposals -- and therefore lists, core distances and MST
+// weights -- carry exact FP32 distances only.
 // Lane layout of the 16x16x32 instruction: A/B lane l holds row (l % 16),
 // k = 8*(l/16) .. +7; the result lane l holds rows 4*(l/16) + q, column l % 16.
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+template <int N>
+__device__ __forceinline__ void merge_short(uint32_t (&a)[N], int xor_mask) {
+  uint32_t o[N];
+#pragma unroll
+  for (int s = 0; s < N; ++s) o[s] = __shfl_xor(a[s], xor_mask);
+#pragma unroll
+  for (int s = 0; s < N; ++s) top_insert32<N>(a, o[s]);
+}
 
 template <bool NEWNEW, int KEEP>
 __global__ __launch_bounds__(64) void k_nnd_join_mfma(const float* __restrict__ X, int64_t n, int d, int dp, int S,
                                                       NndBufs b) {
   constexpr int WPB = 1;
   constexpr int kSlots = KEEP == 0 ? 2 : KEEP;
-  constexpr int kShort = kSlots + 2;
-  extern __shared__ float lds_dyn[];
+  constexpr int kShort = KEEP == 0 ? 2 : KEEP + 2;
+  constexpr uint32_t kNone = 0xFFFFFFFFu;
+  extern __shared__ __attribute__((aligned(16))) float lds_dyn[];
   __shared__ uint32_t s_new[WPB][64];
   __shared__ uint32_t s_old[WPB][64];
   __shared__ uint32_t s_cand[WPB][64];
@@ -648,9 +661,8 @@ __global__ __launch_bounds__(64) void k_nnd_join_mfma(const float* __restrict__ 
   __shared__ float s_norm[64];
   const unsigned lane = __lane_id();
   const int64_t p = blockIdx.x;
-  const int rs = dp + 4;                 // fp32 row stride (16-byte aligned rows)
-  float* vec = lds_dyn;                  // 64 x rs
-  float* gram = lds_dyn + 64 * rs;       // 64 x 65
+  const int rs = dp + 4;  // fp32 row stride (16-byte aligned rows)
+  float* vec = lds_dyn;   // 64 x rs
 
   int nn, nc, rev_drop;
   join_candidates<WPB>(b, p, p < n, S, s_new, s_old, s_cand, &nn, &nc, &rev_drop);
@@ -669,71 +681,94 @@ __global__ __launch_bounds__(64) void k_nnd_join_mfma(const float* __restrict__ 
       }
     }
     for (int k = k0; k < dp; ++k) dst[k] = 0.0f;
-  }
-  __syncthreads();
-  {
     float nrm = 0.0f;
-    for (int k = 0; k < dp; ++k) nrm = __builtin_fmaf(vec[lane * rs + k], vec[lane * rs + k], nrm);
-    s_norm[lane] = nrm;
-  }
-  // Gram tiles (I, J), J >= I, over the live 16-row blocks
-  const int tiles = (nc + 15) >> 4;
-  const int r16 = lane & 15, kq = lane >> 4;
-  for (int I = 0; I < tiles; ++I) {
-    for (int J = I; J < tiles; ++J) {
-      f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
-      for (int k0 = 0; k0 < dp; k0 += 32) {
-        const float* ra = vec + (I * 16 + r16) * rs + k0 + 8 * kq;
-        const float* rb = vec + (J * 16 + r16) * rs + k0 + 8 * kq;
-        bf16x8_t fa, fb;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          fa[e] = static_cast<__bf16>(ra[e]);
-          fb[e] = static_cast<__bf16>(rb[e]);
-        }
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb, acc, 0, 0, 0);
-      }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int gi = I * 16 + 4 * kq + q, gj = J * 16 + r16;
-        gram[gi * 65 + gj] = acc[q];
-        gram[gj * 65 + gi] = acc[q];
-      }
+    for (int k = 0; k < dp; k += 4) {  // dp is a multiple of 32; rows are 16-byte aligned
+      const float4 x = *reinterpret_cast<const float4*>(dst + k);
+      nrm = __builtin_fmaf(x.x, x.x, nrm);
+      nrm = __builtin_fmaf(x.y, x.y, nrm);
+      nrm = __builtin_fmaf(x.z, x.z, nrm);
+      nrm = __builtin_fmaf(x.w, x.w, nrm);
     }
+    s_norm[lane] = nrm;
   }
   __syncthreads();
 
+  const int tiles = (nc + 15) >> 4;
+  const int r16 = static_cast<int>(lane & 15), kq = static_cast<int>(lane >> 4);
+  uint32_t sl[4][kShort];
+#pragma unroll
+  for (int I = 0; I < 4; ++I) {
+#pragma unroll
+    for (int s = 0; s < kShort; ++s) sl[I][s] = kNone;
+    if (I < tiles) {  // uniform across the wave
+      const int row = I * 16 + r16;
+      const bool row_new = row < nn, row_live = row < nc;
+      const float nrow = s_norm[row];
+      for (int J = 0; J < tiles; ++J) {
+        if (I * 16 >= nn && J * 16 >= nn) continue;  // old x old tile: never joined
+        f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int k0 = 0; k0 < dp; k0 += 32) {
+          const float* ra = vec + (J * 16 + r16) * rs + k0 + 8 * kq;
+          const float* rb = vec + (I * 16 + r16) * rs + k0 + 8 * kq;
+          bf16x8_t fa, fb;
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            fa[e] = static_cast<__bf16>(ra[e]);
+            fb[e] = static_cast<__bf16>(rb[e]);
+          }
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int col = J * 16 + 4 * kq + q;
+          const bool ok = row_live && col < nc && col != row &&
+                          (row_new ? (NEWNEW || col >= nn) : (col < nn));
+          const float a = nrow + s_norm[col] - 2.0f * acc[q];
+          const uint32_t key = ok ? ((order_bits(a) & ~63u) | static_cast<uint32_t>(col)) : kNone;
+          if (KEEP == 0) {
+            if (col < nn) sl[I][0] = key < sl[I][0] ? key : sl[I][0];
+            else sl[I][1] = key < sl[I][1] ? key : sl[I][1];
+          } else {
+            top_insert32<kShort>(sl[I], key);
+          }
+        }
+      }
+    }
+  }
+  // the four lanes holding one row (same l % 16) merge their shortlists
+#pragma unroll
+  for (int I = 0; I < 4; ++I) {
+    if (KEEP == 0) {
+#pragma unroll
+      for (int m = 16; m <= 32; m <<= 1) {
+        const uint32_t o0 = __shfl_xor(sl[I][0], m), o1 = __shfl_xor(sl[I][1], m);
+        sl[I][0] = o0 < sl[I][0] ? o0 : sl[I][0];
+        sl[I][1] = o1 < sl[I][1] ? o1 : sl[I][1];
+      }
+    } else {
+      merge_short<kShort>(sl[I], 16);
+      merge_short<kShort>(sl[I], 32);
+    }
+  }
+  // lane c is candidate c: its row's shortlist is sl[c / 16]
+  uint32_t mine[kShort];
+#pragma unroll
+  for (int s = 0; s < kShort; ++s) {
+    const uint32_t v0 = sl[0][s], v1 = sl[1][s], v2 = sl[2][s], v3 = sl[3][s];
+    mine[s] = kq == 0 ? v0 : (kq == 1 ? v1 : (kq == 2 ? v2 : v3));
+  }
+
   const bool active = lane < static_cast<unsigned>(nc);
   const uint32_t my_id = active ? s_cand[0][lane] : 0u;
-  int jb, je;
-  if (static_cast<int>(lane) < nn) {
-    jb = NEWNEW ? 0 : nn;
-    je = nc;
-  } else {
-    jb = 0;
-    je = nn;
-  }
-  // approximate shortlist: key (approx d2 bits, partner index j)
-  uint64_t sl[kShort];
-#pragma unroll
-  for (int s = 0; s < kShort; ++s) sl[s] = kKeyMax;
-  const float ni = s_norm[lane];
-  for (int j = 0; j < nc; ++j) {
-    const bool ok = active && j >= jb && j < je && j != static_cast<int>(lane);
-    float a = ni + s_norm[j] - 2.0f * gram[lane * 65 + j];
-    a = a > 0.0f ? a : 0.0f;
-    const uint64_t key = ok ? ((static_cast<uint64_t>(__float_as_uint(a)) << 32) | static_cast<uint32_t>(j)) : kKeyMax;
-    top_insert<kShort>(sl, key);
-  }
   // exact FP32 re-rank of the shortlist
   uint64_t top[kSlots];
 #pragma unroll
   for (int s = 0; s < kSlots; ++s) top[s] = kKeyMax;
 #pragma unroll
   for (int s = 0; s < kShort; ++s) {
-    if (sl[s] == kKeyMax) continue;
-    const int j = static_cast<int>(static_cast<uint32_t>(sl[s]));
-    const float e = d2_any(vec + lane * rs, vec + j * rs, d);
+    if (!active || mine[s] == kNone) continue;
+    const int j = static_cast<int>(mine[s] & 63u);
+    const float e = (d & 3) == 0 ? d2_vec4(vec + lane * rs, vec + j * rs, d) : d2_any(vec + lane * rs, vec + j * rs, d);
     const uint64_t key = make_key(e, s_cand[0][j], 1u);
     if (KEEP == 0) {
       if (j < nn) top[0] = key < top[0] ? key : top[0];
@@ -1046,7 +1081,7 @@ static int launch_join_k(hdb_ctx* ctx, const float* X, int64_t n, int d, int S, 
     HDB_LAUNCHED();
     return HDB_OK;
   }
-  const int stride = kOwnReg ? ((D + 3) & ~3) : d + 1;
+  const int stride = kOwnReg ? ((D + 3) & ~3) : ((d + 3) & ~3) + 4;
   const size_t lds = static_cast<size_t>(kJoinWaves) * 64 * stride * sizeof(float);
   hipLaunchKernelGGL((k_nnd_join<D, NEWNEW, KEEP>), dim3(grid_for(n, kJoinWaves)), dim3(kJoinWaves * 64), lds,
                      ctx->stream, X, n, d, S, b);
@@ -1068,7 +1103,7 @@ static int launch_join_t(hdb_ctx* ctx, const float* X, int64_t n, int d, int S, 
 template <bool NEWNEW, int KEEP>
 static int launch_join_mfma_k(hdb_ctx* ctx, const float* X, int64_t n, int d, int S, const NndBufs& b) {
   const int dp = (d + 31) & ~31;
-  const size_t lds = (static_cast<size_t>(64) * (dp + 4) + 64 * 65) * sizeof(float);
+  const size_t lds = static_cast<size_t>(64) * (dp + 4) * sizeof(float);
   hipLaunchKernelGGL((k_nnd_join_mfma<NEWNEW, KEEP>), dim3(static_cast<unsigned>(n)), dim3(64), lds, ctx->stream, X,
                      n, d, dp, S, b);
   HDB_LAUNCHED();
