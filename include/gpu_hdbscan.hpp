@@ -11,8 +11,9 @@
 //   - errors throw hdb::Error (the reference prints and calls exit);
 //   - every function takes the hdb::Context to run on (the reference uses
 //     the current CUDA device and the default stream);
-//   - device memory comes from hdb_dev_alloc and is released with
-//     hdb::dev_free (the reference uses cudaMalloc/cudaMallocManaged);
+//   - device memory comes from hdb::Context::alloc (hdb_dev_alloc) and is
+//     released with hdb::Context::free (hdb_dev_free); the reference uses
+//     cudaMalloc/cudaMallocManaged;
 //   - buildEnhancedKNNG does not free its inputs (graph.cu:55,81 do).
 #pragma once
 
