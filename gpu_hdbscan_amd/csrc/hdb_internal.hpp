@@ -28,7 +28,7 @@ enum Slot : int {
   kSlotScanTmp, kSlotRowCnt, kSlotRowCursor, kSlotRowTmp, kSlotRowUniq,  // CSR build
   kSlotAhFlag, kSlotAhAux, kSlotAhKeys, kSlotAhKeys2,                 // anti-hubs
   kSlotMstParent, kSlotMstMin, kSlotMstWl1, kSlotMstWl2, kSlotMstFlag, kSlotMstCnt,
-  kSlotMstSrc, kSlotMstPos, kSlotMstCw, kSlotSeedX, kSlotSeedCand,
+  kSlotMstSrc, kSlotMstPos, kSlotMstCw, kSlotSeedX, kSlotSeedCand, kSlotMlpPack,
   kSlotSortKeys, kSlotSortVals, kSlotSortHist, kSlotSortAux,          // radix sort
   kSlotPipeNindex, kSlotPipeNlist, kSlotPipeW, kSlotPipeAh,  // pipeline
   kSlotBfTmp, kSlotMisc0, kSlotMisc1, kSlotMisc2,
@@ -263,6 +263,31 @@ __device__ __forceinline__ void bitonic128(uint64_t& r0, uint64_t& r1) {
         bool up1 = (k == 128) ? true : (((64u + lane) & k) == 0);
         uint64_t mn0 = r0 < o0 ? r0 : o0, mx0 = r0 < o0 ? o0 : r0;
         uint64_t mn1 = r1 < o1 ? r1 : o1, mx1 = r1 < o1 ? o1 : r1;
+        r0 = (low == up0) ? mn0 : mx0;
+        r1 = (low == up1) ? mn1 : mx1;
+      }
+    }
+  }
+}
+
+// bitonic128 for u32 keys.
+__device__ __forceinline__ void bitonic128_u32(uint32_t& r0, uint32_t& r1) {
+  const unsigned lane = __lane_id();
+#pragma unroll
+  for (int k = 2; k <= 128; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j == 64) {
+        const uint32_t mn = r0 < r1 ? r0 : r1, mx = r0 < r1 ? r1 : r0;
+        r0 = mn;
+        r1 = mx;
+      } else {
+        const uint32_t o0 = __shfl_xor(r0, j), o1 = __shfl_xor(r1, j);
+        const bool low = (lane & j) == 0;
+        const bool up0 = (k == 128) ? true : ((lane & k) == 0);
+        const bool up1 = (k == 128) ? true : (((64u + lane) & k) == 0);
+        const uint32_t mn0 = r0 < o0 ? r0 : o0, mx0 = r0 < o0 ? o0 : r0;
+        const uint32_t mn1 = r1 < o1 ? r1 : o1, mx1 = r1 < o1 ? o1 : r1;
         r0 = (low == up0) ? mn0 : mx0;
         r1 = (low == up1) ? mn1 : mx1;
       }

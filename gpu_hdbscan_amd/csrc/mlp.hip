@@ -49,11 +49,34 @@ static MlpLayout mlp_layout(int kin) {
   return L;
 }
 
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+// GELU x * Phi(x) with the normal tail Q(x) = 0.5 erfc(|x| / sqrt 2) from
+// Abramowitz & Stegun 7.1.26 (|erfc error| <= 1.5e-7): erfc(z) = t (a1 + t
+// (a2 + t (a3 + t (a4 + t a5)))) exp(-z^2), t = 1 / (1 + p z).  Branch-free;
+// exp(-x^2 / 2) is shared with the density used by the derivative.
+struct NormTail {
+  float q;    // Q(|x|)
+  float e;    // exp(-x^2 / 2)
+};
+__device__ __forceinline__ NormTail norm_tail(float x) {
+  // z = |x| / sqrt 2 is folded into p, and the factor 0.5 into a1..a5.
+  const float t = __builtin_amdgcn_rcpf(__builtin_fmaf(0.2316418882663604f, __builtin_fabsf(x), 1.0f));
+  float poly = __builtin_fmaf(t, 0.5307027145f, -0.7265760135f);
+  poly = __builtin_fmaf(t, poly, 0.7107068705f);
+  poly = __builtin_fmaf(t, poly, -0.142248368f);
+  poly = __builtin_fmaf(t, poly, 0.127414796f);
+  NormTail r;
+  r.e = __builtin_amdgcn_exp2f(x * x * -0.7213475204444817f);
+  r.q = t * poly * r.e;
+  return r;
+}
+__device__ __forceinline__ float gelu(float x) {
+  const float q = norm_tail(x).q;
+  return x >= 0.0f ? x - x * q : x * q;
+}
 __device__ __forceinline__ float gelu_grad(float x) {
-  const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
-  const float pdf = 0.39894228040143268f * __expf(-0.5f * x * x);
-  return cdf + x * pdf;
+  const NormTail n = norm_tail(x);
+  const float cdf = x >= 0.0f ? 1.0f - n.q : n.q;
+  return cdf + x * 0.39894228040143268f * n.e;
 }
 
 __device__ __forceinline__ f32x4 mfma(const bf16x8& a, const bf16x8& b, const f32x4& c) {
@@ -451,6 +474,139 @@ __global__ void k_mlp_keys(const float* __restrict__ Z, const float* __restrict_
   ids[i] = static_cast<uint32_t>(i);
 }
 
+// ------------------------------------------------------- inference path --
+// Weights packed into MFMA fragment order (BF16): fragment f holds, for lane
+// l, the 8 weights W[k0 + 8*(l/16) + e][n0 + l%16] (e = 0..7) of column tile
+// n0 and k step k0.  Layer 1 has (kinp/32) x 8 fragments (column tile major),
+// layer 2 4 x 8, layer 3 4 (N = 16).  These are exactly the BF16 operands
+// k_mlp_fwd forms, so both kernels compute the same values.
+__host__ __device__ inline int infer_frags(int kinp) { return (kinp / 32) * 8 + 32 + 4; }
+
+__global__ void k_mlp_pack(const float* __restrict__ prm, MlpLayout L, __bf16* __restrict__ packed) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int ks1 = L.kinp / 32;
+  const int nf1 = ks1 * 8, nf2 = 32;
+  if (t >= static_cast<int64_t>(infer_frags(L.kinp)) * 64) return;
+  int f = static_cast<int>(t >> 6);
+  const int lane = static_cast<int>(t & 63);
+  const float* W;
+  int N, nt, ks;
+  if (f < nf1) {
+    W = prm + L.w1, N = kHid, nt = f / ks1, ks = f % ks1;
+  } else if (f < nf1 + nf2) {
+    f -= nf1;
+    W = prm + L.w2, N = kHid, nt = f / 4, ks = f % 4;
+  } else {
+    f -= nf1 + nf2;
+    W = prm + L.w3, N = kOutP, nt = 0, ks = f;
+  }
+  const int r16 = lane & 15, kq = lane >> 4;
+  for (int e = 0; e < 8; ++e) {
+    const int k = ks * 32 + 8 * kq + e;
+    packed[t * 8 + e] = static_cast<__bf16>(W[static_cast<int64_t>(k) * N + nt * 16 + r16]);
+  }
+}
+
+// Z = forward(X) for R rows: the block keeps the packed weights and biases in
+// LDS and each wave strides over 16-row tiles, with activations stored as
+// BF16 (the value they take as the next layer's MFMA operand).
+__global__ __launch_bounds__(512) void k_mlp_infer(const float* __restrict__ X, int d, int64_t R,
+                                                   const __bf16* __restrict__ packed, const float* __restrict__ prm,
+                                                   MlpLayout L, float* __restrict__ Z) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_b[];
+  const int ks1 = L.kinp / 32;
+  const int nfrag = infer_frags(L.kinp);
+  const int waves = static_cast<int>(blockDim.x >> 6);
+  __bf16* sw = reinterpret_cast<__bf16*>(lds_b);
+  float* sb = reinterpret_cast<float*>(sw + nfrag * 512);  // b1 | b2 | b3: 128 + 128 + 16
+  __bf16* act = reinterpret_cast<__bf16*>(sb + 272);
+  for (int i = threadIdx.x; i < nfrag * 64; i += blockDim.x)
+    reinterpret_cast<uint4*>(sw)[i] = reinterpret_cast<const uint4*>(packed)[i];
+  for (int i = threadIdx.x; i < 272; i += blockDim.x)
+    sb[i] = i < kHid ? prm[L.b1 + i] : (i < 2 * kHid ? prm[L.b2 + i - kHid] : prm[L.b3 + i - 2 * kHid]);
+  __syncthreads();
+  const unsigned l = __lane_id();
+  const int w = static_cast<int>(threadIdx.x >> 6);
+  const int ldi = L.kinp + 8, lda = kHid + 8;
+  __bf16* in = act + static_cast<int64_t>(w) * (16 * ldi + 2 * 16 * lda);
+  __bf16* a1 = in + 16 * ldi;
+  __bf16* a2 = a1 + 16 * lda;
+  const int r16 = static_cast<int>(l & 15), kq = static_cast<int>(l >> 4);
+  const int64_t ntiles = (R + 15) / 16;
+  for (int64_t tile = static_cast<int64_t>(blockIdx.x) * waves + w; tile < ntiles;
+       tile += static_cast<int64_t>(gridDim.x) * waves) {
+    const int64_t r0 = tile * 16;
+    for (int idx = static_cast<int>(l); idx < 16 * L.kinp; idx += 64) {
+      const int i = idx / L.kinp, k = idx - i * L.kinp;
+      const int64_t r = r0 + i;
+      in[i * ldi + k] = static_cast<__bf16>((r < R && k < d) ? X[r * d + k] : 0.0f);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    {  // layer 1: the 8 column tiles interleaved so no MFMA waits on the last
+      f32x4 acc[8];
+#pragma unroll
+      for (int nt = 0; nt < 8; ++nt) acc[nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+      for (int ks = 0; ks < ks1; ++ks) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(in + r16 * ldi + ks * 32 + 8 * kq);
+#pragma unroll
+        for (int nt = 0; nt < 8; ++nt)
+          acc[nt] = mfma(a, *reinterpret_cast<const bf16x8*>(sw + ((nt * ks1 + ks) * 64 + l) * 8), acc[nt]);
+      }
+#pragma unroll
+      for (int nt = 0; nt < 8; ++nt)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int row = 4 * kq + q, col = nt * 16 + r16;
+          a1[row * lda + col] = static_cast<__bf16>(gelu(acc[nt][q] + sb[col]));
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    {  // layer 2
+      f32x4 acc[8];
+#pragma unroll
+      for (int nt = 0; nt < 8; ++nt) acc[nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(a1 + r16 * lda + ks * 32 + 8 * kq);
+#pragma unroll
+        for (int nt = 0; nt < 8; ++nt)
+          acc[nt] = mfma(a, *reinterpret_cast<const bf16x8*>(sw + ((ks1 * 8 + nt * 4 + ks) * 64 + l) * 8), acc[nt]);
+      }
+#pragma unroll
+      for (int nt = 0; nt < 8; ++nt)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int row = 4 * kq + q, col = nt * 16 + r16;
+          a2[row * lda + col] = static_cast<__bf16>(gelu(acc[nt][q] + sb[kHid + col]));
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    {  // layer 3
+      f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(a2 + r16 * lda + ks * 32 + 8 * kq);
+        const bf16x8 b = *reinterpret_cast<const bf16x8*>(sw + ((ks1 * 8 + 32 + ks) * 64 + l) * 8);
+        acc = mfma(a, b, acc);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int64_t row = r0 + 4 * kq + q;
+        if (row < R) Z[row * kOutP + r16] = acc[q] + sb[2 * kHid + r16];
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
 // Points permuted into the bucket order: row r of Xs is point order[r].
 __global__ void k_mlp_permute(const float* __restrict__ X, const uint32_t* __restrict__ order, int64_t n, int d,
                               float* __restrict__ Xs) {
@@ -505,16 +661,28 @@ __global__ __launch_bounds__(256) void k_mlp_seed(const float* __restrict__ Xs, 
     if (r >= n) break;  // uniform across the wave
     const int64_t lo = seed_lo(r, n);
     const float* self_row = rows + (r - sb) * dp;
-    uint64_t k2[2];
+    // rank by 32-bit keys (distance bits with the low 7 replaced by the
+    // window slot), then recompute the winners' exact keys
+    uint32_t k2[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      int64_t p = lo + h * 64 + lane;
+      const int slot = h * 64 + static_cast<int>(lane);
+      int64_t p = lo + slot;
       if (p >= r) p += 1;
-      k2[h] = p < n ? make_key(d2_any(rows + (p - sb) * dp, self_row, d), order[p], 1u) : kKeyMax;
+      k2[h] = p < n ? ((__float_as_uint(d2_any(rows + (p - sb) * dp, self_row, d)) & ~127u) |
+                       static_cast<uint32_t>(slot))
+                    : 0xFFFFFFFFu;
     }
-    bitonic128(k2[0], k2[1]);
-    if (lane < static_cast<unsigned>(kSeedKeep))
-      cand[static_cast<int64_t>(order[r]) * (kSeedTables * kSeedKeep) + table * kSeedKeep + lane] = k2[0];
+    bitonic128_u32(k2[0], k2[1]);
+    if (lane < static_cast<unsigned>(kSeedKeep)) {
+      uint64_t key = kKeyMax;
+      if (k2[0] != 0xFFFFFFFFu) {
+        int64_t p = lo + static_cast<int64_t>(k2[0] & 127u);
+        if (p >= r) p += 1;
+        key = make_key(d2_any(rows + (p - sb) * dp, self_row, d), order[p], 1u);
+      }
+      cand[static_cast<int64_t>(order[r]) * (kSeedTables * kSeedKeep) + table * kSeedKeep + lane] = key;
+    }
   }
 }
 
@@ -581,6 +749,25 @@ int mlp_forward(hdb_ctx* ctx, hdb_mlp* net, const float* X, int d, const int32_t
   else
     hipLaunchKernelGGL(k_mlp_fwd<false>, dim3(grid), dim3(kMlpWaves * 64), fwd_lds(net->L), ctx->stream, X, d, rows, R, net->prm,
                        net->L, P1, H1, P2, H2, Z);
+  HDB_LAUNCHED();
+  return HDB_OK;
+}
+
+// Inference forward Z = f(X) over R rows (packed-weight kernel).
+int mlp_infer(hdb_ctx* ctx, hdb_mlp* net, const float* X, int d, int64_t R, float* Z) {
+  const MlpLayout& L = net->L;
+  const int nfrag = infer_frags(L.kinp);
+  __bf16* packed;
+  HDB_CALL(scratch_t(ctx, kSlotMlpPack, static_cast<int64_t>(nfrag) * 512, &packed));
+  hipLaunchKernelGGL(k_mlp_pack, dim3(grid_for(static_cast<int64_t>(nfrag) * 64, 256)), dim3(256), 0, ctx->stream,
+                     net->prm, L, packed);
+  HDB_LAUNCHED();
+  const int waves = L.kinp <= 64 ? 8 : 4;
+  const size_t lds = static_cast<size_t>(nfrag) * 1024 + 272 * sizeof(float) +
+                     static_cast<size_t>(waves) * (16 * (L.kinp + 8) + 2 * 16 * (kHid + 8)) * 2;
+  const int64_t tiles = (R + 15) / 16;
+  const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>((tiles + waves - 1) / waves, 512)));
+  hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(waves * 64), lds, ctx->stream, X, d, R, packed, net->prm, L, Z);
   HDB_LAUNCHED();
   return HDB_OK;
 }
@@ -753,7 +940,7 @@ int hdb_mlp_forward(hdb_ctx* ctx, hdb_mlp* net, const float* X, int64_t m, int32
   if (!ctx || !net || !X || !Z || m < 0 || d != net->L.kin) return ctx ? ctx->fail(HDB_EINVAL, "bad forward args") : 1;
   HDB_HIP(hipSetDevice(ctx->device));
   if (m == 0) return HDB_OK;
-  return mlp_forward(ctx, net, X, d, nullptr, m, nullptr, nullptr, nullptr, nullptr, Z);
+  return mlp_infer(ctx, net, X, d, m, Z);
 }
 
 int hdb_mlp_loss_grad(hdb_ctx* ctx, hdb_mlp* net, const float* X, int64_t n, int32_t d, const int32_t* rows,
@@ -858,7 +1045,7 @@ int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* const* nets, int32_t tables, const f
   const size_t lds = static_cast<size_t>(kSeedRows) * (d + 1) * sizeof(float);
   const bool staged = lds <= 64 * 1024;
   for (int t = 0; t < tables; ++t) {
-    HDB_CALL(mlp_forward(ctx, nets[t], X, d, nullptr, n, nullptr, nullptr, nullptr, nullptr, Z));
+    HDB_CALL(mlp_infer(ctx, nets[t], X, d, n, Z));
     // x.u range: 64 covers the spread of a unit direction over typical data
     hipLaunchKernelGGL(k_mlp_keys, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, Z, X, d, dirs + t * d, 64.0f,
                        n, keys, ids);

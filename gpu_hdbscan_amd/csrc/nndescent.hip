@@ -78,8 +78,13 @@ __device__ __forceinline__ hdb_nnd_element key_to_elem(uint64_t key) {
   return e;
 }
 
+// Rows read as float4 when every row is 16-byte aligned (same summation
+// order either way, so the same bits).
 __device__ __forceinline__ float point_d2(const float* __restrict__ X, int d, uint32_t a, uint32_t b) {
-  return d2_any(X + static_cast<int64_t>(a) * d, X + static_cast<int64_t>(b) * d, d);
+  const float* ra = X + static_cast<int64_t>(a) * d;
+  const float* rb = X + static_cast<int64_t>(b) * d;
+  if ((d & 3) == 0 && (reinterpret_cast<uintptr_t>(X) & 15) == 0) return d2_vec4(ra, rb, d);
+  return d2_any(ra, rb, d);
 }
 // ------------------------------------------------------------------ init --
 // InitKNNGraphIndexKernel (nndescent.cu:1673-1692) with the xorshift64* index

@@ -39,6 +39,24 @@ def test_forward_and_gradient_match_fp64_oracle(ctx):
     assert np.linalg.norm(g - rg64) / np.linalg.norm(rg64) < 0.1
 
 
+@pytest.mark.parametrize("d", [12, 128])
+def test_forward_grid_stride_and_widths(ctx, d):
+    """The inference kernel strides each wave over 16-row tiles: rows past
+    the first sweep of the grid, and a ragged last tile, give the same
+    outputs as the same rows forwarded alone, and match the FP64 oracle."""
+    print(f"case: mlp inference forward d={d}", flush=True)
+    rng = np.random.default_rng(11 + d)
+    n = 70001
+    X = rng.standard_normal((n, d)).astype(np.float32)
+    net = learned.LearnedInit(ctx, d, seed=7)
+    p = net.get_params()
+    Z = net.forward(X)
+    tail = net.forward(X[-37:])
+    assert np.array_equal(Z[-37:], tail)
+    Zr = M.forward(p.astype(np.float64), X[-200:], d, use_bf16=True)[4]
+    assert np.max(np.abs(Z[-200:] - Zr)) / np.max(np.abs(Zr)) < 2e-2
+
+
 def test_bits_loss_matches_fp64_oracle(ctx):
     print("case: partition-code loss/gradient vs FP64 oracle", flush=True)
     rng = np.random.default_rng(4)

@@ -87,3 +87,30 @@ def test_bits_gradient_matches_finite_differences():
         fd = (M.loss_grad_bits(pp, X, rows, B, codes, d, False)[0] -
               M.loss_grad_bits(pm, X, rows, B, codes, d, False)[0]) / (2 * eps)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (i, fd, g[i])
+
+
+def test_gelu_kernel_constants_match_erf():
+    """The kernels' GELU evaluates the normal tail with the Abramowitz &
+    Stegun 7.1.26 erfc form; its constants, read from the HIP source and
+    evaluated in FP64, must give GELU and its derivative within the
+    published 1.5e-7 bound of the exact-erf forms the oracle uses."""
+    import math
+    import pathlib
+    import re
+    src = (pathlib.Path(__file__).resolve().parents[1] / "gpu_hdbscan_amd" / "csrc" / "mlp.hip").read_text()
+    body = src[src.index("NormTail norm_tail(float x) {"):]
+    body = body[:body.index("return r;")]
+    c = [float(v) for v in re.findall(r"(-?\d+\.\d+)f", body)]
+    p, one, a5, a4, a3, a2, a1, ex = c
+    assert one == 1.0
+    x = np.linspace(-12, 12, 200001)
+    t = 1.0 / (1.0 + p * np.abs(x))
+    poly = (((a5 * t + a4) * t + a3) * t + a2) * t + a1
+    e = np.exp2(x * x * ex)
+    q = t * poly * e
+    gelu = np.where(x >= 0, x - x * q, x * q)
+    cdf = np.where(x >= 0, 1 - q, q)
+    grad = cdf + x * 0.39894228040143268 * e
+    assert np.max(np.abs(gelu - M.gelu(x)) / np.maximum(np.abs(x), 1)) < 1.5e-7
+    assert np.max(np.abs(grad - M.gelu_grad(x)) / np.maximum(np.abs(x), 1)) < 1.5e-7
+    assert abs(ex + 0.5 / math.log(2)) < 1e-12
