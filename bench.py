@@ -13,9 +13,9 @@ brute force; a leg counts only at recall >= --target):
   plain       random init, NND_ITERATION=6, join_keep 4 (the library
               defaults); its kernels supply stage_ms, kernel_ms and roofline
   plain_best  random init at its fastest (join_keep, iterations)
-  learned     learned init (networks trained once on a 200k subsample, the
-              training time reported apart) at its fastest (join_keep,
-              iterations)
+  learned     learned init (6 networks trained once on a 200k subsample, the
+              training time reported apart; window 256) at its fastest
+              (join_keep, iterations)
 value = points / second of the learned leg (of the plain leg when the
 learned leg is skipped or misses the target).
 """
@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="time the plain (random-init) form only")
     ap.add_argument("--learned-subsample", type=int, default=200000)
+    ap.add_argument("--learned-tables", type=int, default=6, help="bucket tables (networks) of the learned init")
+    ap.add_argument("--learned-window", type=int, default=256, help="bucket-order window per point and table")
     ap.add_argument("--target", type=float, default=0.95, help="recall@32 a timed form must reach")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, three device-to-device copies of this size (counter calibration)")
@@ -180,7 +182,8 @@ def main():
                                   "points_per_s": n / (pb[0] * 1e-3), "recall_at_32": round(pb[3], 4)}
         # learned form: networks trained on the GPU from plain NN-Descent over a
         # seeded subsample, then learned initial lists + fewer iterations
-        model = learned.LearnedInit(ctx, d, seed=0).fit(X, n_sub=min(a.learned_subsample, n))
+        model = learned.LearnedInit(ctx, d, seed=0, tables=a.learned_tables,
+                                    window=a.learned_window).fit(X, n_sub=min(a.learned_subsample, n))
         ctx.synchronize()
 
         def learned_step(prm):
@@ -192,6 +195,7 @@ def main():
         if lb:
             legs["learned"] = {"join_keep": lb[1], "iterations": lb[2], "ms_per_step": round(lb[0], 3),
                                "points_per_s": n / (lb[0] * 1e-3), "recall_at_32": round(lb[3], 4),
+                               "tables": a.learned_tables, "window": a.learned_window,
                                "train_s": round(model.train_seconds, 3),
                                "train_subsample": min(a.learned_subsample, n),
                                "final_loss": round(float(model.final_loss), 5),

@@ -235,7 +235,7 @@ int hdb_nndescent(hdb_ctx* ctx, const float* X, int64_t n, int32_t d, const hdb_
   HDB_CALL(check_ctx(ctx));
   if (!params) return ctx->fail(HDB_EINVAL, "null params");
   HDB_CALL(set_device(ctx));
-  return nnd_run(ctx, X, n, d, *params, out, true, params->nnd_iteration);
+  return nnd_run(ctx, X, n, d, *params, out, NndInit::kRandom, params->nnd_iteration);
 }
 
 int hdb_nndescent_from(hdb_ctx* ctx, const float* X, int64_t n, int32_t d, const hdb_params* params,
@@ -243,7 +243,7 @@ int hdb_nndescent_from(hdb_ctx* ctx, const float* X, int64_t n, int32_t d, const
   HDB_CALL(check_ctx(ctx));
   if (!params || iterations < 0) return ctx->fail(HDB_EINVAL, "bad arguments");
   HDB_CALL(set_device(ctx));
-  return nnd_run(ctx, X, n, d, *params, lists, false, iterations);
+  return nnd_run(ctx, X, n, d, *params, lists, NndInit::kGiven, iterations);
 }
 
 int hdb_knn_merge(hdb_ctx* ctx, const float* X, int64_t n1, int64_t n2, int32_t d, const hdb_nnd_element* g1,
@@ -331,9 +331,10 @@ int hdb_pipeline(hdb_ctx* ctx, const float* X, int64_t n, int32_t d, const hdb_p
   {
     Stage st(ctx, "stage_knng");
     if (o.knng_mode == 1) HDB_CALL(nnd_blocked(ctx, X, n, d, o.blocks, *params, lists));
-    else if (o.knng_mode == 2) HDB_CALL(nnd_run(ctx, X, n, d, *params, lists, false, params->nnd_iteration));
-    else if (o.knng_mode == 0) HDB_CALL(nnd_run(ctx, X, n, d, *params, lists, true, params->nnd_iteration));
-    else return ctx->fail(HDB_EINVAL, "knng_mode must be 0, 1 or 2");
+    else if (o.knng_mode == 2) HDB_CALL(nnd_run(ctx, X, n, d, *params, lists, NndInit::kGiven, params->nnd_iteration));
+    else if (o.knng_mode == 3) HDB_CALL(nnd_run(ctx, X, n, d, *params, lists, NndInit::kExact, params->nnd_iteration));
+    else if (o.knng_mode == 0) HDB_CALL(nnd_run(ctx, X, n, d, *params, lists, NndInit::kRandom, params->nnd_iteration));
+    else return ctx->fail(HDB_EINVAL, "knng_mode must be 0, 1, 2 or 3");
   }
   // buildEnhancedKNNG (graph.cu:108-176)
   const int64_t hmax = static_cast<int64_t>(std::sqrt(static_cast<double>(n))) + 2;

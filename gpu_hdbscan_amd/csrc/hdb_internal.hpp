@@ -324,8 +324,12 @@ int read_i64(hdb_ctx* ctx, const int64_t* p, int64_t* out);
 int read_i32(hdb_ctx* ctx, const int32_t* p, int32_t* out);
 
 // nndescent.hip
+// Initial lists: xorshift ids; the caller's ids (distances recomputed,
+// sorted, duplicates dropped); the caller's lists taken as they are (sorted,
+// distinct, exact distances, as hdb_learned_init writes them).
+enum class NndInit { kRandom, kGiven, kExact };
 int nnd_run(hdb_ctx* ctx, const float* X, int64_t n, int d, const hdb_params& prm,
-            hdb_nnd_element* lists, bool init_random, int iterations);
+            hdb_nnd_element* lists, NndInit init, int iterations);
 int nnd_merge_pair(hdb_ctx* ctx, const float* X, int64_t n1, int64_t n2, int d,
                    const hdb_nnd_element* g1, const hdb_nnd_element* g2, const hdb_params& prm,
                    hdb_nnd_element* out);

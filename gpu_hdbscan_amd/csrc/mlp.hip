@@ -617,97 +617,138 @@ __global__ void k_mlp_permute(const float* __restrict__ X, const uint32_t* __res
   Xs[t] = X[static_cast<int64_t>(order[r]) * d + k];
 }
 
-constexpr int kSeedWin = 128;   // bucket-order neighbours examined per point and table
-constexpr int kSeedTables = 4;  // bucket tables (table 0: the trained code bits)
-constexpr int kSeedKeep = 16;   // nearest window entries kept per table
-constexpr int kSeedPts = 32;    // points per block (4 waves x 8)
-constexpr int kSeedRows = kSeedPts + kSeedWin;
-static_assert(kSeedTables * kSeedKeep == 64, "one candidate per lane in the merge");
+constexpr int kSeedTables = 8;  // bucket tables at most
+constexpr int kSeedSlots = 64;  // candidate slots per point (one per lane in the merge)
+constexpr int kSeedPts = 256;   // points per block, one per lane
 
+template <int W>
 __device__ __forceinline__ int64_t seed_lo(int64_t r, int64_t n) {
-  int64_t lo = r - kSeedWin / 2;
-  if (lo > n - kSeedWin - 1) lo = n - kSeedWin - 1;
+  int64_t lo = r - W / 2;
+  if (lo > n - W - 1) lo = n - W - 1;
   return lo < 0 ? 0 : lo;
 }
 
-// One table: the point at sorted position r keeps the kSeedKeep nearest of
-// the kSeedWin points around it in the table's order (exact squared
-// distances, rows staged in LDS) as keys in cand[self][table][.].
-// kStaged: the block's rows sit in LDS (kSeedRows x (d + 1) floats);
-// otherwise they are read from Xs, where they are contiguous.
-template <bool kStaged>
-__global__ __launch_bounds__(256) void k_mlp_seed(const float* __restrict__ Xs, const uint32_t* __restrict__ order,
-                                                  int64_t n, int d, int table, uint64_t* __restrict__ cand) {
+// Lane r (a position in the table's bucket order) keeps the K nearest of
+// the W positions around it (itself skipped), ranked by 32-bit keys: the
+// distance bits with the low log2(W) replaced by the window slot, kept by a
+// compare-exchange insertion network in registers.  The winners' exact keys
+// are recomputed.  Rows come from LDS when STAGED (stride d + 1, an odd
+// stride across lanes); D > 0 fixes d and keeps the point's own row in
+// registers.
+template <int D, int W, int K, bool STAGED>
+__global__ __launch_bounds__(kSeedPts) void k_mlp_seed(const float* __restrict__ Xs,
+                                                        const uint32_t* __restrict__ order, int64_t n, int d,
+                                                        int slot0, uint64_t* __restrict__ cand) {
   extern __shared__ __attribute__((aligned(16))) float lds_rows[];
+  static_assert((W & (W - 1)) == 0 && W <= 256, "window is a power of two");
   const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kSeedPts;
-  const int64_t sb = seed_lo(r0, n);
+  const int64_t sb = seed_lo<W>(r0, n);
   const float* rows = Xs + sb * d;
   int dp = d;
-  if (kStaged) {
-    int64_t se = seed_lo(r0 + kSeedPts - 1, n) + kSeedWin + 1;
+  if (STAGED) {
+    const int64_t last = r0 + kSeedPts - 1 < n ? r0 + kSeedPts - 1 : n - 1;
+    int64_t se = seed_lo<W>(last, n) + W + 1;
     if (se > n) se = n;
     dp = d + 1;
-    for (int64_t t = threadIdx.x; t < (se - sb) * d; t += blockDim.x) {
+    for (int64_t t = threadIdx.x; t < (se - sb) * d; t += kSeedPts) {
       const int64_t rr = t / d;
       lds_rows[rr * dp + (t - rr * d)] = Xs[sb * d + t];
     }
     __syncthreads();
     rows = lds_rows;
   }
-  const unsigned lane = __lane_id();
-  const int wv = static_cast<int>(threadIdx.x >> 6);
-  for (int q = 0; q < kSeedPts / 4; ++q) {
-    const int64_t r = r0 + wv * (kSeedPts / 4) + q;
-    if (r >= n) break;  // uniform across the wave
-    const int64_t lo = seed_lo(r, n);
-    const float* self_row = rows + (r - sb) * dp;
-    // rank by 32-bit keys (distance bits with the low 7 replaced by the
-    // window slot), then recompute the winners' exact keys
-    uint32_t k2[2];
+  const int64_t r = r0 + threadIdx.x;
+  if (r >= n) return;  // after the block's only barrier
+  const int64_t lo = seed_lo<W>(r, n);
+  const float* self_row = rows + (r - sb) * dp;
+  float own[D > 0 ? D : 1];
+  if (D > 0) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int slot = h * 64 + static_cast<int>(lane);
-      int64_t p = lo + slot;
-      if (p >= r) p += 1;
-      k2[h] = p < n ? ((__float_as_uint(d2_any(rows + (p - sb) * dp, self_row, d)) & ~127u) |
-                       static_cast<uint32_t>(slot))
-                    : 0xFFFFFFFFu;
-    }
-    bitonic128_u32(k2[0], k2[1]);
-    if (lane < static_cast<unsigned>(kSeedKeep)) {
-      uint64_t key = kKeyMax;
-      if (k2[0] != 0xFFFFFFFFu) {
-        int64_t p = lo + static_cast<int64_t>(k2[0] & 127u);
-        if (p >= r) p += 1;
-        key = make_key(d2_any(rows + (p - sb) * dp, self_row, d), order[p], 1u);
+    for (int k = 0; k < (D > 0 ? D : 1); ++k) own[k] = self_row[k];
+  }
+  uint32_t best[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) best[j] = 0xFFFFFFFFu;
+  for (int slot = 0; slot < W; ++slot) {
+    int64_t p = lo + slot;
+    if (p >= r) p += 1;
+    if (p >= n) break;  // only when n <= W: the remaining slots are past the end too
+    const float* pr = rows + (p - sb) * dp;
+    float acc = 0.0f;
+    if (D > 0) {
+#pragma unroll
+      for (int k = 0; k < (D > 0 ? D : 1); ++k) {
+        const float diff = pr[k] - own[k];
+        acc = __builtin_fmaf(diff, diff, acc);
       }
-      cand[static_cast<int64_t>(order[r]) * (kSeedTables * kSeedKeep) + table * kSeedKeep + lane] = key;
+    } else {
+      acc = d2_any(pr, self_row, d);
     }
+    uint32_t key = (__float_as_uint(acc) & ~static_cast<uint32_t>(W - 1)) | static_cast<uint32_t>(slot);
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const uint32_t lo_k = key < best[j] ? key : best[j];
+      key = key < best[j] ? best[j] : key;
+      best[j] = lo_k;
+    }
+  }
+  uint64_t* out = cand + static_cast<int64_t>(order[r]) * kSeedSlots + slot0;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    uint64_t key = kKeyMax;
+    if (best[j] != 0xFFFFFFFFu) {
+      int64_t p = lo + static_cast<int64_t>(best[j] & static_cast<uint32_t>(W - 1));
+      if (p >= r) p += 1;
+      key = make_key(d2_any(rows + (p - sb) * dp, self_row, d), order[p], 1u);
+    }
+    out[j] = key;
   }
 }
 
-// Per point: the 32 nearest distinct ids among its table candidates; slots
-// left empty take xorshift ids.
-__global__ __launch_bounds__(256) void k_mlp_merge(const uint64_t* __restrict__ cand, int64_t n, uint64_t seed,
+// Per point: the 32 nearest distinct ids among its table candidates, sorted
+// with exact keys.  Slots no candidate fills take xorshift ids, stepped +1
+// past the point itself and past ids already in the list (n > 33 ids leave
+// room), with their exact distances.
+__global__ __launch_bounds__(256) void k_mlp_merge(const uint64_t* __restrict__ cand, const float* __restrict__ X,
+                                                   int64_t n, int d, uint64_t seed,
                                                    hdb_nnd_element* __restrict__ lists) {
-  const int64_t p = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
-  if (p >= n) return;  // uniform across the wave
+  __shared__ uint64_t s_key[4][64];
+  const unsigned w = threadIdx.x >> 6;
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * 4 + w;
+  if (p >= n) return;  // uniform across the wave; no block barriers below
   const unsigned lane = __lane_id();
-  uint64_t key = cand[p * 64 + lane];
+  uint64_t key = cand[p * kSeedSlots + lane];
   key = bitonic_group<64>(key);
   const uint64_t prev = shfl_u64(key, (lane + 63) & 63);
   const bool keep = key != kKeyMax && (lane == 0 || (prev >> 1) != (key >> 1));
   const uint64_t bal = __ballot(keep);
   const int pos = __popcll(bal & lanemask_lt());
   const int total = __popcll(bal);
-  if (keep && pos < kK)
-    lists[p * kK + pos] = hdb_nnd_element{key_d2(key), static_cast<int32_t>(key_id(key))};
-  if (static_cast<int>(lane) >= total && lane < static_cast<unsigned>(kK)) {
-    uint32_t id = static_cast<uint32_t>(xorshift64star(seed + static_cast<uint64_t>(p) * kK + lane + 1) %
-                                        static_cast<uint64_t>(n));
-    if (id == static_cast<uint32_t>(p)) id = static_cast<uint32_t>((id + 1) % static_cast<uint64_t>(n));
-    lists[p * kK + lane] = hdb_nnd_element{0.0f, static_cast<int32_t>(id)};
+  s_key[w][lane] = kKeyMax;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (keep && pos < kK) s_key[w][pos] = key;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  key = s_key[w][lane];
+  if (total < kK) {  // uniform
+    const float* xp = X + p * d;
+    for (int j = total; j < kK; ++j) {
+      uint32_t id = static_cast<uint32_t>(xorshift64star(seed + static_cast<uint64_t>(p) * kK + j + 1) %
+                                          static_cast<uint64_t>(n));
+      for (;;) {  // at most 33 ids are excluded, so this ends
+        const bool hit = static_cast<int>(lane) < j && key_id(key) == id;
+        if (id != static_cast<uint32_t>(p) && __ballot(hit) == 0) break;
+        id = static_cast<uint32_t>((id + 1) % static_cast<uint64_t>(n));
+      }
+      if (static_cast<int>(lane) == j) key = make_key(d2_any(xp, X + static_cast<int64_t>(id) * d, d), id, 1u);
+    }
+    key = bitonic_group<32>(key);
   }
+  if (lane < static_cast<unsigned>(kK))
+    lists[p * kK + lane] = hdb_nnd_element{key_d2(key), static_cast<int32_t>(key_id(key))};
 }
 
 }  // namespace hdb
@@ -1007,28 +1048,30 @@ int hdb_rp_codes(hdb_ctx* ctx, const float* X, int64_t n, int32_t d, uint64_t se
 }
 
 int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* const* nets, int32_t tables, const float* X, int64_t n, int32_t d,
-                     uint64_t seed, hdb_nnd_element* lists) {
+                     int32_t window, uint64_t seed, hdb_nnd_element* lists) {
   if (!ctx || !nets || tables < 1 || tables > kSeedTables || !X || !lists || n < 34)
-    return ctx ? ctx->fail(HDB_EINVAL, "bad learned_init args (1..4 networks, n must exceed 33)") : 1;
+    return ctx ? ctx->fail(HDB_EINVAL, "bad learned_init args (1..8 networks, n must exceed 33)") : 1;
+  if (window != 128 && window != 256) return ctx->fail(HDB_EINVAL, "window must be 128 or 256");
   for (int t = 0; t < tables; ++t)
     if (!nets[t] || nets[t]->L.kin != d) return ctx->fail(HDB_EINVAL, "network input width differs from d");
   if (n >= (1ll << 31)) return ctx->fail(HDB_EUNSUPPORTED, "more than 2^31 points");
   HDB_HIP(hipSetDevice(ctx->device));
   Stage st(ctx, "learned_init");
+  const int keep = tables <= 4 ? 16 : 8;  // nearest window entries kept per table
   float *Z, *dirs, *Xs;
   uint32_t *keys, *ids;
   uint64_t* cand;
   HDB_CALL(scratch_t(ctx, kSlotMisc0, n * kOutP, &Z));
   HDB_CALL(scratch_t(ctx, kSlotMisc1, n, &keys));
   HDB_CALL(scratch_t(ctx, kSlotMisc2, n, &ids));
-  HDB_CALL(scratch_t(ctx, kSlotBfTmp, static_cast<int64_t>(kSeedTables) * d, &dirs));
+  HDB_CALL(scratch_t(ctx, kSlotBfTmp, static_cast<int64_t>(tables) * d, &dirs));
   HDB_CALL(scratch_t(ctx, kSlotSeedX, n * d, &Xs));
-  HDB_CALL(scratch_t(ctx, kSlotSeedCand, n * kSeedTables * kSeedKeep, &cand));
+  HDB_CALL(scratch_t(ctx, kSlotSeedCand, n * kSeedSlots, &cand));
   // per table: a unit direction for the order inside a bucket, Box-Muller
   // from xorshift64*
-  std::vector<float> hd(static_cast<size_t>(kSeedTables) * d);
+  std::vector<float> hd(static_cast<size_t>(tables) * d);
   uint64_t s = seed * 0x2545F4914F6CDD1Dull + 7;
-  for (int t = 0; t < kSeedTables; ++t) {
+  for (int t = 0; t < tables; ++t) {
     double nrm = 0.0;
     for (int k = 0; k < d; ++k) {
       s = xorshift64star(s);
@@ -1041,9 +1084,10 @@ int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* const* nets, int32_t tables, const f
     for (int k = 0; k < d; ++k) hd[t * d + k] = static_cast<float>(hd[t * d + k] / std::sqrt(nrm));
   }
   HDB_HIP(hipMemcpyAsync(dirs, hd.data(), hd.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-  if (tables < kSeedTables) HDB_CALL(fill_u64(ctx, cand, n * kSeedTables * kSeedKeep, kKeyMax));
-  const size_t lds = static_cast<size_t>(kSeedRows) * (d + 1) * sizeof(float);
-  const bool staged = lds <= 64 * 1024;
+  if (tables * keep < kSeedSlots) HDB_CALL(fill_u64(ctx, cand, n * kSeedSlots, kKeyMax));
+  const size_t lds = static_cast<size_t>(kSeedPts + window + 1) * (d + 1) * sizeof(float);
+  const bool staged = lds <= 80 * 1024;
+  const dim3 grid(static_cast<unsigned>(grid_for(n, kSeedPts)));
   for (int t = 0; t < tables; ++t) {
     HDB_CALL(mlp_infer(ctx, nets[t], X, d, n, Z));
     // x.u range: 64 covers the spread of a unit direction over typical data
@@ -1053,15 +1097,30 @@ int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* const* nets, int32_t tables, const f
     HDB_CALL(radix_sort_u32_u32(ctx, keys, ids, n, 0, 32));
     hipLaunchKernelGGL(k_mlp_permute, dim3(grid_for(n * d, 256)), dim3(256), 0, ctx->stream, X, ids, n, d, Xs);
     HDB_LAUNCHED();
-    if (staged)
-      hipLaunchKernelGGL(k_mlp_seed<true>, dim3(grid_for(n, kSeedPts)), dim3(256), lds, ctx->stream, Xs, ids, n, d, t,
-                         cand);
-    else
-      hipLaunchKernelGGL(k_mlp_seed<false>, dim3(grid_for(n, kSeedPts)), dim3(256), 0, ctx->stream, Xs, ids, n, d, t,
-                         cand);
+    const int slot0 = t * keep;
+#define HDB_SEED(D_, W_, K_, ST_)                                                                                   \
+  hipLaunchKernelGGL((k_mlp_seed<D_, W_, K_, ST_>), grid, dim3(kSeedPts), ST_ ? lds : 0, ctx->stream, Xs, ids, n, d, \
+                     slot0, cand)
+#define HDB_SEED_K(D_, W_, ST_)                 \
+  do {                                          \
+    if (keep == 16) HDB_SEED(D_, W_, 16, ST_);  \
+    else HDB_SEED(D_, W_, 8, ST_);              \
+  } while (0)
+#define HDB_SEED_W(D_, ST_)                                \
+  do {                                                     \
+    if (window == 128) HDB_SEED_K(D_, 128, ST_);           \
+    else HDB_SEED_K(D_, 256, ST_);                         \
+  } while (0)
+    if (d == 12 && staged) HDB_SEED_W(12, true);
+    else if (d == 32 && staged) HDB_SEED_W(32, true);
+    else if (staged) HDB_SEED_W(0, true);
+    else HDB_SEED_W(0, false);
+#undef HDB_SEED_W
+#undef HDB_SEED_K
+#undef HDB_SEED
     HDB_LAUNCHED();
   }
-  hipLaunchKernelGGL(k_mlp_merge, dim3(grid_for(n, 4)), dim3(256), 0, ctx->stream, cand, n, seed, lists);
+  hipLaunchKernelGGL(k_mlp_merge, dim3(grid_for(n, 4)), dim3(256), 0, ctx->stream, cand, X, n, d, seed, lists);
   HDB_LAUNCHED();
   return HDB_OK;
 }

@@ -159,6 +159,20 @@ __global__ __launch_bounds__(256) void k_nnd_init_given(const float* __restrict_
   }
 }
 
+// Lists already sorted, distinct and exact (as hdb_learned_init writes
+// them): every entry is marked new and the list bound read off.
+__global__ __launch_bounds__(256) void k_nnd_init_exact(hdb_nnd_element* __restrict__ lists, int64_t n,
+                                                        float* __restrict__ worst) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= n * kK) return;
+  hdb_nnd_element e = lists[t];
+  if (e.label < 0) {
+    e.label = -e.label - 1;
+    lists[t] = e;
+  }
+  if ((t & (kK - 1)) == kK - 1) worst[t / kK] = e.distance;
+}
+
 // knnmerge init: InitRandomBlockedKNNGraph (gpuknn/knnmerge.cu:80-185) after
 // MarkAllToOld of both inputs (:487-503).  List p keeps the first 16 entries
 // of its own graph (old; ids of graph 2 shifted by n1) and draws 16 new
@@ -1203,7 +1217,7 @@ static int check_params(hdb_ctx* ctx, const hdb_params& prm, int64_t n, int d) {
 }
 
 int nnd_run(hdb_ctx* ctx, const float* X, int64_t n, int d, const hdb_params& prm, hdb_nnd_element* lists,
-            bool init_random, int iterations) {
+            NndInit init, int iterations) {
   HDB_CALL(check_params(ctx, prm, n, d));
   if (!X || !lists) return ctx->fail(HDB_EINVAL, "null pointer");
   if (n <= kK + 1 && iterations > 0) {
@@ -1218,12 +1232,14 @@ int nnd_run(hdb_ctx* ctx, const float* X, int64_t n, int d, const hdb_params& pr
   HDB_CALL(get_bufs(ctx, n, prm, &b));
   {
     Stage st(ctx, "nnd_init");
-    if (init_random)
+    if (init == NndInit::kRandom)
       hipLaunchKernelGGL(k_nnd_init_random, dim3(grid_for(n * kK, 256)), dim3(256), 0, ctx->stream, X, n, d, lists,
                          b.worst);
-    else
+    else if (init == NndInit::kGiven)
       hipLaunchKernelGGL(k_nnd_init_given, dim3(grid_for(n * kK, 256)), dim3(256), 0, ctx->stream, X, n, d, lists,
                          b.worst);
+    else
+      hipLaunchKernelGGL(k_nnd_init_exact, dim3(grid_for(n * kK, 256)), dim3(256), 0, ctx->stream, lists, n, b.worst);
     HDB_LAUNCHED();
   }
   HDB_CALL(iterate(ctx, X, n, d, prm.sample_num, false, 0, iterations, lists, b));
@@ -1273,7 +1289,7 @@ int nnd_blocked(hdb_ctx* ctx, const float* X, int64_t n, int d, int blocks, cons
   // local ids; the running lists start as the block graphs in global ids.
   for (int i = 0; i < blocks; ++i) {
     const int64_t ni = off[i + 1] - off[i];
-    HDB_CALL(nnd_run(ctx, X + off[i] * d, ni, d, prm, local + off[i] * kK, true, prm.nnd_iteration));
+    HDB_CALL(nnd_run(ctx, X + off[i] * d, ni, d, prm, local + off[i] * kK, NndInit::kRandom, prm.nnd_iteration));
     for (int s = 0; s < 4; ++s) stats[s] += ctx->stats_host[s];
     hipLaunchKernelGGL(k_offset_labels, dim3(grid_for(ni * kK, 256)), dim3(256), 0, ctx->stream, local + off[i] * kK,
                        out + off[i] * kK, ni * kK, off[i]);

@@ -22,15 +22,18 @@ from ._lib import NND_ELEMENT
 
 
 class LearnedInit:
-    """`tables` networks (1..4), each trained to reproduce its own balanced
-    random-projection-tree partition of a subsample while keeping the
-    subsample's kNN pairs in one bucket; their sign codes are the bucket
-    tables of the initial lists."""
+    """`tables` networks (1..8); each network's 12 sign bits are one bucket
+    table of the initial lists (see `fit` for the two training losses).
+    `window`: bucket-order positions examined per point and table (128 or
+    256)."""
 
-    def __init__(self, ctx: api.Context, d: int, seed: int = 0, tables: int = 4):
-        if not 1 <= tables <= 4:
-            raise ValueError("tables must be in 1..4")
+    def __init__(self, ctx: api.Context, d: int, seed: int = 0, tables: int = 6, window: int = 256):
+        if not 1 <= tables <= 8:
+            raise ValueError("tables must be in 1..8")
+        if window not in (128, 256):
+            raise ValueError("window must be 128 or 256")
         self.ctx = ctx
+        self.window = int(window)
         self.d = int(d)
         self.seed = int(seed)
         self.handles = []
@@ -133,7 +136,7 @@ class LearnedInit:
         L = out if out is not None else self.ctx.empty((n, 32), NND_ELEMENT)
         arr = (C.c_void_p * len(self.handles))(*self.handles)
         self.ctx._check(self.ctx.lib.hdb_learned_init(self.ctx.handle, arr, len(self.handles), Xd.ptr, n, self.d,
-                                                      C.c_uint64(seed), L.ptr))
+                                                      self.window, C.c_uint64(seed), L.ptr))
         return L
 
 
@@ -167,14 +170,14 @@ def sampled_recall(ctx: api.Context, Xd: api.DeviceArray, lists: api.DeviceArray
 def pipeline_learned(ctx: api.Context, X, model: LearnedInit, params=None, outputs: dict | None = None,
                      keep_on_device: bool = True) -> api.PipelineResult:
     """The whole path with learned initial lists: hdb_learned_init, then
-    hdb_pipeline in knng_mode 2 (params.nnd_iteration NN-Descent iterations
-    from those lists), on the context stream."""
+    hdb_pipeline in knng_mode 3 (params.nnd_iteration NN-Descent iterations
+    from those lists, taken as written), on the context stream."""
     Xd = ctx.dev(X, np.float32)
     bufs = dict(outputs or {})
     if bufs.get("lists") is None:
         bufs["lists"] = ctx.empty((Xd.shape[0], 32), NND_ELEMENT)
     model.init_lists(Xd, out=bufs["lists"])
-    return api.pipeline(ctx, Xd, params=params, knng_mode=2, keep_on_device=keep_on_device, outputs=bufs)
+    return api.pipeline(ctx, Xd, params=params, knng_mode=3, keep_on_device=keep_on_device, outputs=bufs)
 
 
 def nndescent_learned(ctx: api.Context, X, model: LearnedInit, iterations: int, target: float = 0.95,

@@ -190,7 +190,11 @@ typedef struct {
   int32_t knng_mode;      /* 0: one NN-Descent over all points (random init);
                              1: blocked all-pairs;
                              2: NN-Descent from the lists already in lists_dev
-                                (e.g. written by hdb_learned_init) */
+                                (ids taken, distances recomputed);
+                             3: NN-Descent from lists exactly as
+                                hdb_learned_init writes them (sorted,
+                                distinct, exact squared distances; taken
+                                as they are) */
   int32_t blocks;         /* knng_mode 1: number of blocks */
   int32_t keep_csr;       /* nonzero: leave the MRD CSR in the context (testing) */
   int32_t reserved[5];
@@ -246,13 +250,14 @@ int hdb_mlp_train(hdb_ctx* ctx, hdb_mlp* net, const float* X_dev, int64_t n, int
  * cell.  These are the training targets of the partition-code loss. */
 int hdb_rp_codes(hdb_ctx* ctx, const float* X_dev, int64_t n, int32_t d, uint64_t seed,
                  uint32_t* codes_dev);
-/* Initial lists from 1..4 trained networks (ids with exact squared
+/* Initial lists from 1..8 trained networks (ids with exact squared
  * distances; hdb_nndescent_from recomputes them): network t's 12 sign bits
  * are table t's bucket code, points are ordered by (code, x . u_t), and
- * each point keeps the 16 nearest of the 128 points around it in each
- * table; the 32 nearest distinct ids of all tables form its list. */
+ * each point keeps the nearest 16 (tables <= 4) or 8 (more tables) of the
+ * `window` (128 or 256) points around it in each table; the 32 nearest
+ * distinct ids of all tables form its list. */
 int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* const* nets, int32_t tables, const float* X_dev,
-                     int64_t n, int32_t d, uint64_t seed, hdb_nnd_element* lists_dev);
+                     int64_t n, int32_t d, int32_t window, uint64_t seed, hdb_nnd_element* lists_dev);
 
 /* ---- host labels (HDBSCAN/trees/tree.cu) ----------------------------------- */
 /* Single linkage -> condensed tree (min cluster size mpts) -> stability ->

@@ -99,6 +99,48 @@ def test_rp_codes_balanced_and_local(ctx):
     assert same_knn > 20 * same_rnd
 
 
+@pytest.mark.parametrize("tables,window,d", [(1, 128, 12), (3, 256, 20), (8, 128, 12), (8, 256, 32)])
+def test_learned_init_lists_valid(ctx, tables, window, d):
+    """Initial lists from 1..8 tables and both windows: ids in range, never
+    the point itself, distinct per list, sorted, with exact FP32 distances
+    (the oracle's fused-multiply-add order), random fills included."""
+    print(f"case: learned init lists tables={tables} window={window} d={d}", flush=True)
+    rng = np.random.default_rng(tables * 1000 + window + d)
+    n = 5000
+    X = (rng.standard_normal((n, d)) * rng.uniform(0.5, 3.0, d)).astype(np.float32)
+    net = learned.LearnedInit(ctx, d, seed=2, tables=tables, window=window)
+    L = net.init_lists(ctx.to_device(X)).numpy()
+    lab, dist = L["label"].astype(np.int64), L["distance"]
+    assert lab.min() >= 0 and lab.max() < n
+    assert not (lab == np.arange(n)[:, None]).any()
+    assert all(len(np.unique(r)) == 32 for r in lab)
+    assert np.all(np.diff(dist, axis=1) >= 0)
+    for i in range(0, n, 7):
+        for j in range(32):
+            assert dist[i, j] == O.d2(X[i], X[lab[i, j]]), (i, j)
+
+
+def test_pipeline_exact_lists_mode_matches_given_mode(ctx):
+    """knng_mode 3 takes hdb_learned_init's lists as written; mode 2
+    recomputes their distances, sorts and deduplicates them.  On those lists
+    the two starts agree, so every output of the path is identical."""
+    print("case: pipeline knng_mode 3 vs 2 on learned lists 20000x12", flush=True)
+    X = datasets.make("mix30_1m_d12", n=20000, seed=3)
+    Xd = ctx.to_device(X)
+    net = learned.LearnedInit(ctx, 12, seed=3, tables=3, window=128)
+    outs = []
+    for mode in (2, 3):
+        L = net.init_lists(Xd)
+        r = api.pipeline(ctx, Xd, params=api.default_params(nnd_iteration=3), knng_mode=mode,
+                         outputs={"lists": L})
+        outs.append(r)
+    a, b = outs
+    assert np.array_equal(a.lists, b.lists)
+    assert np.array_equal(a.counts, b.counts)
+    assert np.array_equal(a.core.view(np.uint32), b.core.view(np.uint32))
+    assert a.n_edges == b.n_edges and np.array_equal(a.edges, b.edges)
+
+
 def test_training_and_learned_init(ctx):
     print("case: learned init aniso200 60000x32", flush=True)
     X = datasets.make("aniso200_5m_d32", n=60000, seed=4)
