@@ -396,11 +396,47 @@ __global__ void k_seg_bounds(const uint32_t* __restrict__ keys, int64_t m, int64
   if (i == m - 1 || (keys[i + 1] >> shift) != k) e[k] = static_cast<int32_t>(i + 1);
 }
 
+// Four keys per lane from one 16-byte load; the neighbours across lanes
+// come by shuffle, and only the wave's first and last lanes read the keys
+// just outside their four.  keys must be 16-byte aligned.
+__global__ void k_seg_bounds4(const uint32_t* __restrict__ keys, int64_t m, int64_t nseg, int shift,
+                              int32_t* __restrict__ s, int32_t* __restrict__ e) {
+  const int64_t i0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * 4;
+  const unsigned lane = __lane_id();
+  uint32_t k[4];
+  if (i0 + 3 < m) {
+    const uint4 v = *reinterpret_cast<const uint4*>(keys + i0);
+    k[0] = v.x, k[1] = v.y, k[2] = v.z, k[3] = v.w;
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) k[j] = i0 + j < m ? keys[i0 + j] : 0u;
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) k[j] >>= shift;
+  uint32_t prev = __shfl_up(k[3], 1);
+  uint32_t next = __shfl_down(k[0], 1);
+  if (lane == 0 && i0 > 0 && i0 - 1 < m) prev = keys[i0 - 1] >> shift;
+  if (lane == 63 && i0 + 4 < m) next = keys[i0 + 4] >> shift;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int64_t i = i0 + j;
+    if (i >= m || static_cast<int64_t>(k[j]) >= nseg) continue;
+    const uint32_t before = j == 0 ? prev : k[j - 1];
+    const uint32_t after = j == 3 ? next : k[j + 1];
+    if (i == 0 || before != k[j]) s[k[j]] = static_cast<int32_t>(i);
+    if (i == m - 1 || after != k[j]) e[k[j]] = static_cast<int32_t>(i + 1);
+  }
+}
+
 int segment_bounds(hdb_ctx* ctx, const uint32_t* keys, int64_t m, int64_t nseg, int32_t* s, int32_t* e, int shift) {
   HDB_CALL(fill_i32(ctx, s, nseg, 0));
   HDB_CALL(fill_i32(ctx, e, nseg, 0));
   if (m <= 0) return HDB_OK;
-  hipLaunchKernelGGL(k_seg_bounds, dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream, keys, m, nseg, shift, s, e);
+  if ((reinterpret_cast<uintptr_t>(keys) & 15) == 0)
+    hipLaunchKernelGGL(k_seg_bounds4, dim3(grid_for((m + 3) / 4, 256)), dim3(256), 0, ctx->stream, keys, m, nseg,
+                       shift, s, e);
+  else
+    hipLaunchKernelGGL(k_seg_bounds, dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream, keys, m, nseg, shift, s, e);
   HDB_LAUNCHED();
   return HDB_OK;
 }
