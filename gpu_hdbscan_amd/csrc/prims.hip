@@ -177,22 +177,44 @@ __device__ __forceinline__ int64_t rs_phys(const RsSegs& sg, const int64_t* s_pr
   return static_cast<int64_t>(lo) * sg.seg_cap + (i - s_pre[lo]);
 }
 
+// Physical indices of one wave's 64-key chunk starting at logical c0 (lane
+// adds its lane index): the segment is found once per chunk (wave-uniform
+// search); a lane past that segment's end searches on its own.
+__device__ __forceinline__ int64_t rs_phys_chunk(const RsSegs& sg, const int64_t* s_pre, int64_t c0, unsigned lane) {
+  const int64_t i = c0 + lane;
+  if (!sg.pre) return i;
+  int lo = 0, hi = sg.nseg;
+  while (hi - lo > 1) {  // uniform: c0 is the same in every lane
+    const int mid = (lo + hi) >> 1;
+    if (s_pre[mid] <= c0) lo = mid; else hi = mid;
+  }
+  if (lo + 1 < sg.nseg && i >= s_pre[lo + 1]) return rs_phys(sg, s_pre, i);
+  return static_cast<int64_t>(lo) * sg.seg_cap + (i - s_pre[lo]);
+}
+
 template <typename K>
-__global__ void k_rs_hist(const K* __restrict__ keys, int64_t n, int shift, int64_t ntiles,
-                          int32_t* __restrict__ hist, RsSegs sg) {
+__global__ __launch_bounds__(kRsBlock) void k_rs_hist(const K* __restrict__ keys, int64_t n, int shift,
+                                                      int64_t ntiles, int32_t* __restrict__ hist, RsSegs sg) {
   __shared__ int32_t h[kRsBins];
   __shared__ int64_t s_pre[kRsMaxSegs + 1];
   for (int i = threadIdx.x; i < kRsBins; i += kRsBlock) h[i] = 0;
   rs_load_segs(sg, s_pre);
   __syncthreads();
-  const int64_t base = static_cast<int64_t>(blockIdx.x) * kRsTile;
-  for (int i = threadIdx.x; i < kRsTile; i += kRsBlock) {
-    int64_t idx = base + i;
-    if (idx < n) {
-      unsigned dgt = static_cast<unsigned>(keys[rs_phys(sg, s_pre, idx)] >> shift) & 0xFFu;
-      atomicAdd(&h[dgt], 1);
-    }
+  const unsigned lane = __lane_id();
+  const int64_t wbase = static_cast<int64_t>(blockIdx.x) * kRsTile + static_cast<int64_t>(threadIdx.x >> 6) * kRsWaveKeys;
+  // every chunk's load is issued before the first is used
+  int64_t ph[kRsChunks];
+#pragma unroll
+  for (int c = 0; c < kRsChunks; ++c) {
+    const int64_t c0 = wbase + c * 64;
+    ph[c] = c0 + lane < n ? rs_phys_chunk(sg, s_pre, c0, lane) : -1;
   }
+  K kv[kRsChunks];
+#pragma unroll
+  for (int c = 0; c < kRsChunks; ++c) kv[c] = keys[ph[c] >= 0 ? ph[c] : 0];
+#pragma unroll
+  for (int c = 0; c < kRsChunks; ++c)
+    if (ph[c] >= 0) atomicAdd(&h[static_cast<unsigned>(kv[c] >> shift) & 0xFFu], 1);
   __syncthreads();
   for (int i = threadIdx.x; i < kRsBins; i += kRsBlock) hist[static_cast<int64_t>(i) * ntiles + blockIdx.x] = h[i];
 }
@@ -225,13 +247,26 @@ __global__ __launch_bounds__(kRsBlock) void k_rs_scatter(const K* __restrict__ k
   int32_t rank[kRsChunks];
   uint8_t dig[kRsChunks];
   const uint64_t lt = lanemask_lt();
+  // addresses first, then every chunk's loads back to back (no branch
+  // between them), so their latencies overlap
+  {
+    int64_t ph[kRsChunks];
+#pragma unroll
+    for (int c = 0; c < kRsChunks; ++c) {
+      const int64_t c0 = wbase + c * 64;
+      ph[c] = c0 + lane < n ? rs_phys_chunk(sg, s_pre, c0, lane) : 0;
+    }
+#pragma unroll
+    for (int c = 0; c < kRsChunks; ++c) key[c] = kin[ph[c]];
+    if (HAS_V) {
+#pragma unroll
+      for (int c = 0; c < kRsChunks; ++c) val[c] = vin[ph[c]];
+    }
+  }
 #pragma unroll
   for (int c = 0; c < kRsChunks; ++c) {
     const int64_t idx = wbase + c * 64 + lane;
     const bool act = idx < n;
-    const int64_t ph = act ? rs_phys(sg, s_pre, idx) : 0;
-    key[c] = act ? kin[ph] : K(0);
-    if (HAS_V) val[c] = act ? vin[ph] : V(0);
     const unsigned dg = static_cast<unsigned>(key[c] >> shift) & 0xFFu;
     dig[c] = static_cast<uint8_t>(dg);
     uint64_t m = __ballot(act);
