@@ -41,36 +41,56 @@ __device__ __forceinline__ int find_compress(int v, int* __restrict__ parent) {
 // Lightest edge leaving the component of each live vertex (sixteen lanes
 // per row; the grid strides over groups of sixteen vertices).  A vertex
 // whose row has no such edge stays dead: components only grow.  Offers the
-// weight to the component (cw); one atomic per block adds the block's count
-// of vertices with an edge to `active` (same-address atomics serialise, so
-// there are few of them).
+// weight to the component (cw); a block with such a vertex sets `active`
+// to 1 with a plain store (no same-address atomics, which serialise).
 __global__ __launch_bounds__(256) void k_bv_best(const int64_t* __restrict__ nindex, const int32_t* __restrict__ nlist,
                                                  const float* __restrict__ w, int64_t n, const int32_t* __restrict__ comp,
                                                  uint32_t* __restrict__ best_w, uint64_t* __restrict__ best_p,
                                                  uint32_t* __restrict__ cw, unsigned long long* __restrict__ active) {
-  __shared__ unsigned long long s_cnt;
-  if (threadIdx.x == 0) s_cnt = 0;
+  __shared__ int s_any;
+  if (threadIdx.x == 0) s_any = 0;
   __syncthreads();
   const int sl = static_cast<int>(threadIdx.x & 15);
-  unsigned long long mine = 0;
+  bool mine = false;
   for (int64_t g = blockIdx.x; g * 16 < n; g += gridDim.x) {
     const int64_t v = g * 16 + (threadIdx.x >> 4);
     uint32_t bw = kNoEdge;
     uint64_t bp = ~0ull;
     int rv = 0;
-    const bool live = v < n && best_w[v] != kNoEdge;
+    const int64_t vc = v < n ? v : n - 1;
+    const bool live = v < n && best_w[vc] != kNoEdge;
+    const int64_t e0 = nindex[vc], e1 = nindex[vc + 1];
     if (live) {
       rv = comp[v];
-      const int64_t e1 = nindex[v + 1];
-      for (int64_t j = nindex[v] + sl; j < e1; j += 16) {
-        const int32_t u = nlist[j];
-        if (u < 0 || static_cast<int64_t>(u) >= n || comp[u] == rv) continue;
-        const uint32_t wb = wkey(w[j]);
-        const uint64_t pr = v < u ? ((static_cast<uint64_t>(v) << 32) | static_cast<uint32_t>(u))
-                                  : ((static_cast<uint64_t>(u) << 32) | static_cast<uint64_t>(v));
-        if (wb < bw || (wb == bw && pr < bp)) {
-          bw = wb;
-          bp = pr;
+      // four entries per lane per step: their loads, then their
+      // components' loads, are issued together
+      for (int64_t j = e0 + sl; j < e1; j += 64) {
+        int32_t u[4];
+        float wt[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int64_t jj = j + 16 * t < e1 ? j + 16 * t : j;
+          u[t] = j + 16 * t < e1 ? nlist[jj] : -1;
+          wt[t] = w[jj];
+        }
+        int cu[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const bool ok = u[t] >= 0 && static_cast<int64_t>(u[t]) < n;
+          cu[t] = comp[ok ? u[t] : v];
+          if (!ok) cu[t] = rv;
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          if (cu[t] == rv) continue;
+          const uint32_t wb = wkey(wt[t]);
+          const int64_t uu = u[t];
+          const uint64_t pr = v < uu ? ((static_cast<uint64_t>(v) << 32) | static_cast<uint32_t>(uu))
+                                     : ((static_cast<uint64_t>(uu) << 32) | static_cast<uint64_t>(v));
+          if (wb < bw || (wb == bw && pr < bp)) {
+            bw = wb;
+            bp = pr;
+          }
         }
       }
     }
@@ -88,13 +108,13 @@ __global__ __launch_bounds__(256) void k_bv_best(const int64_t* __restrict__ nin
       best_p[v] = bp;
       if (bw != kNoEdge) {
         atomicMin(&cw[rv], bw);
-        ++mine;
+        mine = true;
       }
     }
   }
-  if (mine) atomicAdd(&s_cnt, mine);
+  if (mine) s_any = 1;
   __syncthreads();
-  if (threadIdx.x == 0 && s_cnt) atomicAdd(active, s_cnt);
+  if (threadIdx.x == 0 && s_any) *active = 1ull;  // a plain store: only zero / nonzero is read
 }
 
 // Among the vertices that offered the component's weight, the smallest pair.
@@ -267,7 +287,7 @@ int mst_csr(hdb_ctx* ctx, const int64_t* nindex, const int32_t* nlist, const flo
   HDB_CALL(scratch_t(ctx, kSlotMstCnt, 4, &cnt));
   const unsigned gv = grid_for(n, 256);
   // fixed grids for the kernels that end in a same-address atomic per block
-  const unsigned gs = static_cast<unsigned>(std::min<int64_t>(grid_for(n, 16), 2048));
+  const unsigned gs = static_cast<unsigned>(std::min<int64_t>(grid_for(n, 16), 16384));
   const unsigned gh = static_cast<unsigned>(std::min<int64_t>(gv, 1024));
   hipLaunchKernelGGL(k_iota, dim3(gv), dim3(256), 0, ctx->stream, comp, n);
   HDB_LAUNCHED();
