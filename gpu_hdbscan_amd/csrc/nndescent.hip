@@ -925,9 +925,26 @@ __global__ __launch_bounds__(256) void k_nnd_update(hdb_nnd_element* __restrict_
       if (c == 0) continue;  // uniform across the wave
       uint64_t cur = s_cur[q][lane & 31u];
       cur = lane < 32 ? cur : kKeyMax;
+      const uint64_t* lst = s_cur[q];  // the list as it stands, sorted, in LDS
       for (int c0 = 0; c0 < c; c0 += 64) {
-        const uint64_t pr = c0 + static_cast<int>(lane) < c ? s_key[s_start[q] + c0 + lane] : kKeyMax;
+        uint64_t pr = c0 + static_cast<int>(lane) < c ? s_key[s_start[q] + c0 + lane] : kKeyMax;
+        if (!full) {
+          // After the first iteration lists are distinct, so a proposal that
+          // ranks past the 32nd entry, or equals an entry's (distance, id),
+          // cannot change the list: drop it, and skip the merge when none is
+          // left.
+          const uint64_t pk = pr >> 1;
+          int pos = 0;  // entries below the proposal, capped at 31
+#pragma unroll
+          for (int step = 16; step > 0; step >>= 1)
+            if ((lst[pos + step - 1] >> 1) < pk) pos += step;
+          // lst[pos] >= pk unless all 32 entries are below it (pos = 31)
+          const bool keep = pr != kKeyMax && (lst[pos] >> 1) > pk;
+          if (__ballot(keep) == 0) continue;
+          pr = keep ? pr : kKeyMax;
+        }
         cur = merge_proposals(cur, pr, s_prop[w], s_top[w]);
+        lst = s_top[w];
       }
       if (lane < 32) s_cur[q][lane] = cur;
       if (lane == 0) s_hit[q] = 1;
