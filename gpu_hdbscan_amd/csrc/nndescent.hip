@@ -857,6 +857,42 @@ __device__ __forceinline__ uint64_t merge_proposals(uint64_t cur, uint64_t pr, u
   return lane < 32 ? tv : kKeyMax;
 }
 
+// The list in lanes 0..31 (sorted, distinct) with the proposals `live`
+// marks (distinct from the list entries, each at `pos` = # entries below
+// it): positions come from counting, with no sort.  The lane holding the
+// first of equal proposals keeps it.  Returns the new list in lanes 0..31.
+__device__ __forceinline__ uint64_t insert_few(uint64_t cur, uint64_t pr, bool live, int pos, uint64_t* s_top) {
+  const unsigned lane = __lane_id();
+  uint64_t lm = __ballot(live);
+  bool dup = false;
+  for (uint64_t m = lm; m; m &= m - 1) {  // uniform loop over the proposals
+    const int t = __builtin_ctzll(m);
+    const uint64_t x = (static_cast<uint64_t>(__builtin_amdgcn_readlane(static_cast<uint32_t>(pr >> 32), t)) << 32) |
+                       __builtin_amdgcn_readlane(static_cast<uint32_t>(pr), t);
+    dup = dup || (t < static_cast<int>(lane) && (x >> 1) == (pr >> 1));
+  }
+  live = live && !dup;
+  lm = __ballot(live);
+  int below_cur = 0, below_pr = 0;
+  for (uint64_t m = lm; m; m &= m - 1) {
+    const int t = __builtin_ctzll(m);
+    const uint64_t x = (static_cast<uint64_t>(__builtin_amdgcn_readlane(static_cast<uint32_t>(pr >> 32), t)) << 32) |
+                       __builtin_amdgcn_readlane(static_cast<uint32_t>(pr), t);
+    below_cur += x < cur ? 1 : 0;
+    below_pr += x < pr ? 1 : 0;
+  }
+  const int pc = static_cast<int>(lane) + below_cur;
+  if (lane < 32u && pc < kK) s_top[pc] = cur;
+  const int pp = pos + below_pr;
+  if (live && pp < kK) s_top[pp] = pr;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  const uint64_t tv = s_top[lane & 31u];
+  __builtin_amdgcn_wave_barrier();
+  return lane < 32 ? tv : kKeyMax;
+}
+
 constexpr int kGroupBits = 4;     // lists per update block: 16
 constexpr int kGroupTile = 1024;  // proposals staged per LDS tile
 
@@ -940,7 +976,13 @@ __global__ __launch_bounds__(256) void k_nnd_update(hdb_nnd_element* __restrict_
             if ((lst[pos + step - 1] >> 1) < pk) pos += step;
           // lst[pos] >= pk unless all 32 entries are below it (pos = 31)
           const bool keep = pr != kKeyMax && (lst[pos] >> 1) > pk;
-          if (__ballot(keep) == 0) continue;
+          const uint64_t km = __ballot(keep);
+          if (km == 0) continue;
+          if (__popcll(km) <= 32) {
+            cur = insert_few(cur, pr, keep, pos, s_top[w]);
+            lst = s_top[w];
+            continue;
+          }
           pr = keep ? pr : kKeyMax;
         }
         cur = merge_proposals(cur, pr, s_prop[w], s_top[w]);
