@@ -857,30 +857,27 @@ __device__ __forceinline__ uint64_t merge_proposals(uint64_t cur, uint64_t pr, u
   return lane < 32 ? tv : kKeyMax;
 }
 
-// The list in lanes 0..31 (sorted, distinct) with the proposals `live`
-// marks (distinct from the list entries, each at `pos` = # entries below
-// it): positions come from counting, with no sort.  The lane holding the
-// first of equal proposals keeps it.  Returns the new list in lanes 0..31.
+// The list in lanes 0..31 (sorted, distinct) merged with the proposals
+// `live` marks (each distinct from the list's entries, `pos` = # entries
+// below it): every element's new position is counted, with no sort.  Of
+// equal proposals the lowest lane's is kept.  Returns the new list in lanes
+// 0..31; s_top is the wave's 32-entry LDS row.
 __device__ __forceinline__ uint64_t insert_few(uint64_t cur, uint64_t pr, bool live, int pos, uint64_t* s_top) {
   const unsigned lane = __lane_id();
-  uint64_t lm = __ballot(live);
-  bool dup = false;
-  for (uint64_t m = lm; m; m &= m - 1) {  // uniform loop over the proposals
+  bool dup = false;  // an equal proposal sits in a lower lane
+  int below_cur = 0, below_pr = 0;
+  // uniform loop over the proposals in lane order: when step t is reached,
+  // lane t has met every lower lane, so its dup flag is final
+  for (uint64_t m = __ballot(live); m; m &= m - 1) {
     const int t = __builtin_ctzll(m);
     const uint64_t x = (static_cast<uint64_t>(__builtin_amdgcn_readlane(static_cast<uint32_t>(pr >> 32), t)) << 32) |
                        __builtin_amdgcn_readlane(static_cast<uint32_t>(pr), t);
+    if (__builtin_amdgcn_readlane(dup ? 1u : 0u, t)) continue;  // uniform
+    below_cur += x < cur ? 1 : 0;
+    below_pr += x < pr ? 1 : 0;
     dup = dup || (t < static_cast<int>(lane) && (x >> 1) == (pr >> 1));
   }
   live = live && !dup;
-  lm = __ballot(live);
-  int below_cur = 0, below_pr = 0;
-  for (uint64_t m = lm; m; m &= m - 1) {
-    const int t = __builtin_ctzll(m);
-    const uint64_t x = (static_cast<uint64_t>(__builtin_amdgcn_readlane(static_cast<uint32_t>(pr >> 32), t)) << 32) |
-                       __builtin_amdgcn_readlane(static_cast<uint32_t>(pr), t);
-    below_cur += x < cur ? 1 : 0;
-    below_pr += x < pr ? 1 : 0;
-  }
   const int pc = static_cast<int>(lane) + below_cur;
   if (lane < 32u && pc < kK) s_top[pc] = cur;
   const int pp = pos + below_pr;
@@ -978,14 +975,12 @@ __global__ __launch_bounds__(256) void k_nnd_update(hdb_nnd_element* __restrict_
           const bool keep = pr != kKeyMax && (lst[pos] >> 1) > pk;
           const uint64_t km = __ballot(keep);
           if (km == 0) continue;
-          if (__popcll(km) <= 32) {
-            cur = insert_few(cur, pr, keep, pos, s_top[w]);
-            lst = s_top[w];
-            continue;
-          }
-          pr = keep ? pr : kKeyMax;
+          // counting costs a step per proposal; past about 32 the sort wins
+          if (__popcll(km) <= 32) cur = insert_few(cur, pr, keep, pos, s_top[w]);
+          else cur = merge_proposals(cur, keep ? pr : kKeyMax, s_prop[w], s_top[w]);
+        } else {
+          cur = merge_proposals(cur, pr, s_prop[w], s_top[w]);
         }
-        cur = merge_proposals(cur, pr, s_prop[w], s_top[w]);
         lst = s_top[w];
       }
       if (lane < 32) s_cur[q][lane] = cur;
