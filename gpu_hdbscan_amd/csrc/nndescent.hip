@@ -1199,8 +1199,11 @@ static int launch_join(hdb_ctx* ctx, const float* X, int64_t n, int d, int S, co
   return launch_join_t<0, NEWNEW>(ctx, X, n, d, S, b);
 }
 
+// first_full: the first update merges every list, which also removes
+// repeated ids (random and knnmerge starts may hold them); lists that start
+// distinct skip that.
 static int iterate(hdb_ctx* ctx, const float* X, int64_t n, int d, int S, bool merge, int64_t split, int iterations,
-                   hdb_nnd_element* lists, NndBufs& b) {
+                   hdb_nnd_element* lists, NndBufs& b, bool first_full = true) {
   const int rev_bits = bits_for(2 * n);
   const int tgt_bits = bits_for(n - 1);
   const int64_t ngroups = (n + (1 << kGroupBits) - 1) >> kGroupBits;
@@ -1246,7 +1249,7 @@ static int iterate(hdb_ctx* ctx, const float* X, int64_t n, int d, int S, bool m
     {
       Stage st(ctx, merge ? "nnd_update_merge" : "nnd_update");
       hipLaunchKernelGGL(k_nnd_update, dim3(static_cast<unsigned>(ngroups)), dim3(256), 0, ctx->stream, lists, n, bu,
-                         it == 0 ? 1 : 0);
+                         it == 0 && first_full ? 1 : 0);
       HDB_LAUNCHED();
     }
   }
@@ -1296,7 +1299,7 @@ int nnd_run(hdb_ctx* ctx, const float* X, int64_t n, int d, const hdb_params& pr
       hipLaunchKernelGGL(k_nnd_init_exact, dim3(grid_for(n * kK, 256)), dim3(256), 0, ctx->stream, lists, n, b.worst);
     HDB_LAUNCHED();
   }
-  HDB_CALL(iterate(ctx, X, n, d, prm.sample_num, false, 0, iterations, lists, b));
+  HDB_CALL(iterate(ctx, X, n, d, prm.sample_num, false, 0, iterations, lists, b, init == NndInit::kRandom));
   {
     Stage st(ctx, "nnd_finalize");
     hipLaunchKernelGGL(k_nnd_finalize, dim3(grid_for(n * kK, 256)), dim3(256), 0, ctx->stream, lists, n * kK);
