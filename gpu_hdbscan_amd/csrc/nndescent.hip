@@ -541,7 +541,7 @@ __device__ __forceinline__ void top_insert32(uint32_t (&top)[SLOTS], uint32_t ke
 // new candidate by |a|^2 + |b|^2 - 2 a.b (so each old-new pair is evaluated
 // once) and old candidates store that value in an LDS block; phase 2: new
 // candidates read their old partners from that block.  Ranking uses 32-bit
-// keys -- the order-preserving bits of that value with the low 6 bits
+// keys -- the bits of that value, clamped at 0, with the low 6 bits
 // replaced by the partner's index -- and the KEEP winners' exact squared
 // distances are then recomputed from the same rows in the same operation
 // order as k_nnd_join, so proposals carry exactly its distances; only the
@@ -606,10 +606,12 @@ __global__ __launch_bounds__(kJoinWaves * 64) void k_nnd_join2(const float* __re
     float dot = 0.0f;
 #pragma unroll
     for (int k = 0; k < D; ++k) dot = __builtin_fmaf(own[k], row[k], dot);
-    const float acc = __builtin_fmaf(-2.0f, dot, own_norm + s_norm[w][j]);
+    // clamped at 0 (cancellation can dip below): non-negative float bits
+    // order like the values, so the key needs no sign fix-up
+    const float acc = fmaxf(__builtin_fmaf(-2.0f, dot, own_norm + s_norm[w][j]), 0.0f);
     if (active && !is_new) s_blk[w][j * bst + (static_cast<int>(lane) - nn)] = acc;
     const bool ok = active && j != static_cast<int>(lane) && (NEWNEW || !is_new);
-    const uint32_t key = ok ? ((order_bits(acc) & ~63u) | static_cast<uint32_t>(j)) : kNone;
+    const uint32_t key = ok ? ((__float_as_uint(acc) & ~63u) | static_cast<uint32_t>(j)) : kNone;
     if (KEEP == 0) top[0] = key < top[0] ? key : top[0];
     else top_insert32<kSlots>(top, key);
   }
@@ -617,7 +619,7 @@ __global__ __launch_bounds__(kJoinWaves * 64) void k_nnd_join2(const float* __re
   // phase 2: new candidates take their old partners from the block
   for (int o = 0; o < no; ++o) {
     const float acc = is_new ? s_blk[w][lane * bst + o] : 0.0f;
-    const uint32_t key = is_new ? ((order_bits(acc) & ~63u) | static_cast<uint32_t>(nn + o)) : kNone;
+    const uint32_t key = is_new ? ((__float_as_uint(acc) & ~63u) | static_cast<uint32_t>(nn + o)) : kNone;
     if (KEEP == 0) top[1] = key < top[1] ? key : top[1];
     else top_insert32<kSlots>(top, key);
   }
