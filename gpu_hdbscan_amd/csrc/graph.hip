@@ -201,9 +201,11 @@ __global__ __launch_bounds__(256) void k_row_count(const hdb_nnd_element* __rest
                                                    int64_t h, int32_t* __restrict__ own_sorted,
                                                    int32_t* __restrict__ uown, int32_t* __restrict__ urev,
                                                    int32_t* __restrict__ total) {
+  __shared__ uint32_t s_own[4][64];
   const unsigned lane = __lane_id();
-  const int64_t v = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
-  if (v >= n) return;  // uniform across the wave
+  const unsigned w = threadIdx.x >> 6;
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * 4 + w;
+  if (v >= n) return;  // uniform across the wave; no block barriers below
   const uint64_t lt = lanemask_lt();
   uint64_t x = kKeyMax;
   if (static_cast<int>(lane) < k) {
@@ -215,7 +217,13 @@ __global__ __launch_bounds__(256) void k_row_count(const hdb_nnd_element* __rest
   const bool ko = x != kKeyMax && (lane == 0 || px != x);
   const uint64_t bo = __ballot(ko);
   const int uo = __popcll(bo);
-  if (ko) own_sorted[v * k + __popcll(bo & lt)] = static_cast<int32_t>(x);
+  if (ko) {
+    own_sorted[v * k + __popcll(bo & lt)] = static_cast<int32_t>(x);
+    s_own[w][__popcll(bo & lt)] = static_cast<uint32_t>(x);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   const int64_t b0 = rs[v], b1 = re[v];
   int64_t ur = 0;
   uint32_t carry = 0xFFFFFFFFu;
@@ -225,12 +233,14 @@ __global__ __launch_bounds__(256) void k_row_count(const hdb_nnd_element* __rest
     uint32_t py = __shfl(y, static_cast<int>((lane + 63) & 63));
     if (lane == 0) py = carry;
     bool keep = p < b1 && y != py && static_cast<int64_t>(y) != v;
-    // drop ids that are own entries (mutual kNN pairs are listed once)
-    for (uint64_t m = bo; m; m &= m - 1) {
-      const int l = __builtin_ctzll(m);
-      const uint32_t o = static_cast<uint32_t>(shfl_u64(x, l));
-      keep = keep && y != o;
+    // drop ids that are own entries (mutual kNN pairs are listed once):
+    // binary search in the sorted own ids (uo is the same in every lane)
+    int lo = 0, hi = uo;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (s_own[w][mid] < y) lo = mid + 1; else hi = mid;
     }
+    keep = keep && !(lo < uo && s_own[w][lo] == y);
     const uint64_t bk = __ballot(keep);
     if (keep) rev[b0 + ur + __popcll(bk & lt)] = y;
     ur += __popcll(bk);
@@ -266,13 +276,26 @@ __global__ __launch_bounds__(256) void k_row_fill(const float* __restrict__ X, i
                                                   const int64_t* __restrict__ nindex, const float* __restrict__ core,
                                                   const int32_t* __restrict__ ahrank, const int32_t* __restrict__ ah,
                                                   int64_t h, int32_t* __restrict__ nlist, float* __restrict__ w) {
+  constexpr int kRevLds = 192;  // reverse entries staged in LDS when the row has at most this many
+  __shared__ uint32_t s_own[4][64];
+  __shared__ uint32_t s_rev[4][kRevLds];
   const unsigned lane = __lane_id();
-  const int64_t v = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
-  if (v >= n) return;
-  const int32_t* own = own_sorted + v * k;
+  const unsigned wv = threadIdx.x >> 6;
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * 4 + wv;
+  if (v >= n) return;  // uniform across the wave; no block barriers below
   const int64_t uo = uown[v];
-  const uint32_t* rv = rev + rs[v];
   const int64_t ur = urev[v];
+  // the row's own and reverse ids (both sorted) go to LDS, so the rank
+  // searches below read LDS; a longer reverse run is searched in place
+  const uint32_t* rvg = rev + rs[v];
+  if (static_cast<int64_t>(lane) < uo) s_own[wv][lane] = static_cast<uint32_t>(own_sorted[v * k + lane]);
+  if (ur <= kRevLds)
+    for (int64_t b = lane; b < ur; b += 64) s_rev[wv][b] = rvg[b];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  const uint32_t* own = s_own[wv];
+  const uint32_t* rv = ur <= kRevLds ? s_rev[wv] : rvg;
   const int64_t out0 = nindex[v];
   const float cv = core[v];
   const int32_t r = ahrank[v];
