@@ -207,14 +207,15 @@ __global__ __launch_bounds__(256) void k_row_count(const hdb_nnd_element* __rest
   const int64_t v = static_cast<int64_t>(blockIdx.x) * 4 + w;
   if (v >= n) return;  // uniform across the wave; no block barriers below
   const uint64_t lt = lanemask_lt();
-  uint64_t x = kKeyMax;
+  constexpr uint32_t kNoId = 0xFFFFFFFFu;  // ids are below 2^31
+  uint32_t x = kNoId;
   if (static_cast<int>(lane) < k) {
     const uint32_t id = label_id(lists[v * k + lane].label);
     if (id != kInvalidId && static_cast<int64_t>(id) < n && static_cast<int64_t>(id) != v) x = id;
   }
-  x = bitonic_group<64>(x);
-  const uint64_t px = shfl_u64(x, (lane + 63) & 63);
-  const bool ko = x != kKeyMax && (lane == 0 || px != x);
+  x = bitonic_group_u32<64>(x);
+  const uint32_t px = __shfl(x, static_cast<int>((lane + 63) & 63));
+  const bool ko = x != kNoId && (lane == 0 || px != x);
   const uint64_t bo = __ballot(ko);
   const int uo = __popcll(bo);
   if (ko) {

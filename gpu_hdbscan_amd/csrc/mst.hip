@@ -58,28 +58,46 @@ __global__ __launch_bounds__(256) void k_bv_best(const int64_t* __restrict__ nin
     uint64_t bp = ~0ull;
     int rv = 0;
     const int64_t vc = v < n ? v : n - 1;
-    const bool live = v < n && best_w[vc] != kNoEdge;
+    const uint32_t pw = best_w[vc];
+    const uint64_t pp = best_p[vc];  // meaningful only when pw is a weight
     const int64_t e0 = nindex[vc], e1 = nindex[vc + 1];
-    if (live) {
-      rv = comp[v];
-      // four entries per lane per step: their loads, then their
-      // components' loads, are issued together
-      for (int64_t j = e0 + sl; j < e1; j += 64) {
-        int32_t u[4];
-        float wt[4];
+    const int rvv = comp[vc];
+    const bool live = v < n && pw != kNoEdge;
+    // components only grow, so a vertex's external edges only disappear: a
+    // previous lightest edge that is still external is still the lightest
+    // (pw == 0: none yet).  Its check and the row's first entries are loaded
+    // together.
+    const bool had = live && pw != 0u;
+    int64_t other = static_cast<int64_t>(pp >> 32) == vc ? static_cast<int64_t>(pp & 0xFFFFFFFFu)
+                                                          : static_cast<int64_t>(pp >> 32);
+    if (!had || other < 0 || other >= n) other = vc;
+    const int co = comp[other];
+    // four entries per lane per step: their loads, then their components'
+    // loads, are issued together
+    int32_t u[4] = {-1, -1, -1, -1};
+    float wt[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (live && e1 > e0) {
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          const int64_t jj = j + 16 * t < e1 ? j + 16 * t : j;
-          u[t] = j + 16 * t < e1 ? nlist[jj] : -1;
-          wt[t] = w[jj];
-        }
-        int cu[4];
+      for (int t = 0; t < 4; ++t) {
+        const int64_t jj = e0 + sl + 16 * t;
+        const int64_t jc = jj < e1 ? jj : e0;
+        u[t] = jj < e1 ? nlist[jc] : -1;
+        wt[t] = w[jc];
+      }
+    }
+    int cu[4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          const bool ok = u[t] >= 0 && static_cast<int64_t>(u[t]) < n;
-          cu[t] = comp[ok ? u[t] : v];
-          if (!ok) cu[t] = rv;
-        }
+    for (int t = 0; t < 4; ++t) {
+      const bool ok = u[t] >= 0 && static_cast<int64_t>(u[t]) < n;
+      cu[t] = comp[ok ? u[t] : vc];
+      if (!ok) cu[t] = rvv;
+    }
+    if (live) rv = rvv;
+    if (had && co != rvv) {
+      bw = pw;
+      bp = pp;
+    } else if (live) {
+      for (int64_t j = e0 + sl;;) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
           if (cu[t] == rv) continue;
@@ -91,6 +109,21 @@ __global__ __launch_bounds__(256) void k_bv_best(const int64_t* __restrict__ nin
             bw = wb;
             bp = pr;
           }
+        }
+        j += 64;
+        if (j >= e1) break;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int64_t jj = j + 16 * t;
+          const int64_t jc = jj < e1 ? jj : j;
+          u[t] = jj < e1 ? nlist[jc] : -1;
+          wt[t] = w[jc];
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const bool ok = u[t] >= 0 && static_cast<int64_t>(u[t]) < n;
+          cu[t] = comp[ok ? u[t] : vc];
+          if (!ok) cu[t] = rv;
         }
       }
     }
@@ -107,7 +140,10 @@ __global__ __launch_bounds__(256) void k_bv_best(const int64_t* __restrict__ nin
       best_w[v] = bw;
       best_p[v] = bp;
       if (bw != kNoEdge) {
-        atomicMin(&cw[rv], bw);
+        // cw only falls during the kernel, so a read (even a stale one) that
+        // is not above bw means the atomic cannot lower it: members of a
+        // large component mostly skip the same-address atomic
+        if (bw < cw[rv]) atomicMin(&cw[rv], bw);
         mine = true;
       }
     }
