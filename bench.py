@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="mix30_1m_d12")
     ap.add_argument("--iterations", type=int, default=6)
+    ap.add_argument("--distance-mode", choices=("fp32", "bf16"), default="fp32",
+                    help="bf16: BF16 MFMA candidate tiles with exact FP32 re-rank (the 2M x 128 variant)")
     ap.add_argument("--blocked", type=int, default=0, help="blocked all-pairs kNNG with this many blocks")
     ap.add_argument("--queries", type=int, default=10000)
     ap.add_argument("--cpu-sample", type=int, default=100000)
@@ -148,7 +150,7 @@ def main():
         return recall_at_32(ctx, Xd, bufs["lists"], q, gt_ids)
 
     # plain form: random init, the reference's NND_ITERATION
-    p_plain = api.default_params(nnd_iteration=a.iterations)
+    p_plain = api.default_params(nnd_iteration=a.iterations, distance_mode=1 if a.distance_mode == "bf16" else 0)
 
     def plain_step(prm=p_plain):
         return api.pipeline(ctx, Xd, params=prm, knng_mode=mode, blocks=max(a.blocked, 1), keep_on_device=True,
@@ -238,7 +240,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "bf16 tiles + f32" if a.distance_mode == "bf16" else "f32",
         "data": "synthetic",
         "config": {"workload": a.workload, "points": n, "dim": d, "k": 32, "sample_num": 16,
                    "nnd_iteration": legs.get("learned", {}).get("iterations", iters) if init_used == "learned"
