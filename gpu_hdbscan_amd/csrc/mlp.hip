@@ -266,33 +266,55 @@ __global__ void k_mlp_reduce(const float* __restrict__ part, int64_t splits, int
   grad[i] = s;
 }
 
-// Bias gradients, first pass: column sums of rows [z*chunk, (z+1)*chunk) of
-// a dense R x n matrix into part[z][c] (block z, one thread per column).
-__global__ void k_mlp_colsum(const float* __restrict__ G, int64_t R, int n, int64_t chunk, float* __restrict__ part) {
-  const int c = threadIdx.x;
-  if (c >= n) return;
-  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * chunk;
-  const int64_t r1 = r0 + chunk < R ? r0 + chunk : R;
-  float s = 0.0f;
-  for (int64_t r = r0; r < r1; ++r) s += G[r * n + c];
-  part[static_cast<int64_t>(blockIdx.x) * n + c] = s;
+// Bias gradients, first pass: column sums of rows [z*256, (z+1)*256) of a
+// dense R x n matrix into part[z][c] (block z; n divides 256, so 256 / n
+// threads share a column, each with four running sums over its rows, then
+// combined in a fixed order).
+__global__ __launch_bounds__(256) void k_mlp_colsum(const float* __restrict__ G, int64_t R, int n,
+                                                    float* __restrict__ part) {
+  __shared__ float acc[256];
+  const int t = static_cast<int>(threadIdx.x);
+  const int c = t % n, h = t / n, hp = 256 / n;
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * 256;
+  float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int64_t r = r0 + h; r < r0 + 256 && r < R; r += 4 * hp) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t ru = r + u * hp;
+      if (ru < r0 + 256 && ru < R) s[u] += G[ru * n + c];
+    }
+  }
+  acc[t] = (s[0] + s[1]) + (s[2] + s[3]);
+  __syncthreads();
+  if (h == 0) {
+    float tot = 0.0f;
+    for (int q = 0; q < hp; ++q) tot += acc[q * n + c];
+    part[static_cast<int64_t>(blockIdx.x) * n + c] = tot;
+  }
 }
 
-// Mean of tanh(z) per code bit over the R rows (one block).
-__global__ __launch_bounds__(256) void k_mlp_tmean(const float* __restrict__ Z, int64_t R, float* __restrict__ mean) {
+// Mean of tanh(z) per code bit over the R rows: block z sums rows
+// [z*256, (z+1)*256) into part[z][b]; k_mlp_tmean_final adds the parts in
+// order and divides by R.
+__global__ __launch_bounds__(256) void k_mlp_tmean(const float* __restrict__ Z, int64_t R, float* __restrict__ part) {
   __shared__ float acc[kCode][256];
-  float s[kCode];
-  for (int b = 0; b < kCode; ++b) s[b] = 0.0f;
-  for (int64_t r = threadIdx.x; r < R; r += 256)
-    for (int b = 0; b < kCode; ++b) s[b] += tanhf(Z[r * kOutP + b]);
-  for (int b = 0; b < kCode; ++b) acc[b][threadIdx.x] = s[b];
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  for (int b = 0; b < kCode; ++b) acc[b][threadIdx.x] = r < R ? tanhf(Z[r * kOutP + b]) : 0.0f;
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
     if (static_cast<int>(threadIdx.x) < st)
       for (int b = 0; b < kCode; ++b) acc[b][threadIdx.x] += acc[b][threadIdx.x + st];
     __syncthreads();
   }
-  if (threadIdx.x < kCode) mean[threadIdx.x] = acc[threadIdx.x][0] / static_cast<float>(R);
+  if (threadIdx.x < kCode) part[static_cast<int64_t>(blockIdx.x) * kCode + threadIdx.x] = acc[threadIdx.x][0];
+}
+
+__global__ void k_mlp_tmean_final(const float* __restrict__ part, int64_t blocks, int64_t R, float* __restrict__ mean) {
+  const int b = static_cast<int>(threadIdx.x);
+  if (b >= kCode) return;
+  float s = 0.0f;
+  for (int64_t z = 0; z < blocks; ++z) s += part[z * kCode + b];
+  mean[b] = s / static_cast<float>(R);
 }
 
 // Loss and dZ for B (anchor, positive, negative) triples stored as rows
@@ -830,10 +852,9 @@ int wgrad(hdb_ctx* ctx, const float* A, int lda, const float* X, int d, const in
 // Column sums of an R x n matrix (n <= 128) into out, in a fixed order:
 // 256-row partial sums, then their sum.  part holds ceil(R/256) * n floats.
 int colsum(hdb_ctx* ctx, const float* G, int64_t R, int n, float* part, float* out) {
-  const int64_t chunk = 256;
-  const int64_t splits = (R + chunk - 1) / chunk;
-  hipLaunchKernelGGL(k_mlp_colsum, dim3(static_cast<unsigned>(splits)), dim3(kHid), 0, ctx->stream, G, R, n, chunk,
-                     part);
+  if (n < 1 || 256 % n != 0) return ctx->fail(HDB_EINVAL, "column count must divide 256");
+  const int64_t splits = (R + 255) / 256;
+  hipLaunchKernelGGL(k_mlp_colsum, dim3(static_cast<unsigned>(splits)), dim3(256), 0, ctx->stream, G, R, n, part);
   HDB_LAUNCHED();
   hipLaunchKernelGGL(k_mlp_reduce, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, part, splits,
                      static_cast<int64_t>(n), out);
@@ -864,7 +885,9 @@ int mlp_loss_grad(hdb_ctx* ctx, hdb_mlp* net, const float* X, int d, const int32
     hipLaunchKernelGGL(k_mlp_loss_bits, dim3(grid_for(R, 256)), dim3(256), 0, ctx->stream, Z, B, rows, codes, dZ,
                        lossp);
   } else {
-    hipLaunchKernelGGL(k_mlp_tmean, dim3(1), dim3(256), 0, ctx->stream, Z, R, mean);
+    hipLaunchKernelGGL(k_mlp_tmean, dim3(static_cast<unsigned>(splits)), dim3(256), 0, ctx->stream, Z, R, part);
+    HDB_LAUNCHED();
+    hipLaunchKernelGGL(k_mlp_tmean_final, dim3(1), dim3(64), 0, ctx->stream, part, splits, R, mean);
     HDB_LAUNCHED();
     hipLaunchKernelGGL(k_mlp_loss, dim3(grid_for(B, 256)), dim3(256), 0, ctx->stream, Z, B, mean, dZ, lossp);
   }
