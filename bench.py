@@ -13,8 +13,8 @@ brute force; a leg counts only at recall >= --target):
   plain       random init, NND_ITERATION=6, join_keep 4 (the library
               defaults); its kernels supply stage_ms, kernel_ms and roofline
   plain_best  random init at its fastest (join_keep, iterations)
-  learned     learned init (6 networks trained once on a 200k subsample, the
-              training time reported apart; window 256) at its fastest
+  learned     learned init (8 networks trained once on a 200k subsample, the
+              training time reported apart; window 512) at its fastest
               (join_keep, iterations)
 value = points / second of the learned leg (of the plain leg when the
 learned leg is skipped or misses the target).
@@ -50,8 +50,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="time the plain (random-init) form only")
     ap.add_argument("--learned-subsample", type=int, default=200000)
-    ap.add_argument("--learned-tables", type=int, default=6, help="bucket tables (networks) of the learned init")
-    ap.add_argument("--learned-window", type=int, default=256, help="bucket-order window per point and table")
+    ap.add_argument("--learned-tables", type=int, default=8, help="bucket tables (networks) of the learned init")
+    ap.add_argument("--learned-window", type=int, default=512, help="bucket-order window per point and table")
     ap.add_argument("--target", type=float, default=0.95, help="recall@32 a timed form must reach")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, three device-to-device copies of this size (counter calibration)")

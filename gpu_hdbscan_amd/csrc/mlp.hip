@@ -662,7 +662,7 @@ __global__ __launch_bounds__(kSeedPts) void k_mlp_seed(const float* __restrict__
                                                         const uint32_t* __restrict__ order, int64_t n, int d,
                                                         int slot0, uint64_t* __restrict__ cand) {
   extern __shared__ __attribute__((aligned(16))) float lds_rows[];
-  static_assert((W & (W - 1)) == 0 && W <= 256, "window is a power of two");
+  static_assert((W & (W - 1)) == 0 && W <= 512, "window is a power of two");
   const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kSeedPts;
   const int64_t sb = seed_lo<W>(r0, n);
   const float* rows = Xs + sb * d;
@@ -722,6 +722,90 @@ __global__ __launch_bounds__(kSeedPts) void k_mlp_seed(const float* __restrict__
       int64_t p = lo + static_cast<int64_t>(best[j] & static_cast<uint32_t>(W - 1));
       if (p >= r) p += 1;
       key = make_key(d2_any(rows + (p - sb) * dp, self_row, d), order[p], 1u);
+    }
+    out[j] = key;
+  }
+}
+
+// k_mlp_seed for d = D in {12, 32}: rows staged at a stride of D floats when
+// D / 4 is odd, else D + 4 (16-byte rows whose starts fall in distinct
+// 4-bank groups for 16 consecutive lanes), read as float4, and ranked by
+// |x|^2 + |y|^2 - 2 x.y (clamped at 0) with the rows' norms staged beside
+// them; the winners' exact keys are recomputed as in k_mlp_seed.
+template <int D, int W, int K>
+__global__ __launch_bounds__(kSeedPts) void k_mlp_seed4(const float* __restrict__ Xs,
+                                                         const uint32_t* __restrict__ order, int64_t n,
+                                                         int slot0, uint64_t* __restrict__ cand) {
+  static_assert(D % 4 == 0 && D <= 32, "float4 rows");
+  static_assert((W & (W - 1)) == 0 && W <= 512, "window is a power of two");
+  constexpr int S = ((D / 4) & 1) ? D : D + 4;
+  extern __shared__ __attribute__((aligned(16))) float lds_rows[];
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kSeedPts;
+  const int64_t sb = seed_lo<W>(r0, n);
+  const int64_t last = r0 + kSeedPts - 1 < n ? r0 + kSeedPts - 1 : n - 1;
+  int64_t se = seed_lo<W>(last, n) + W + 1;
+  if (se > n) se = n;
+  const int nrows = static_cast<int>(se - sb);
+  float* s_norm = lds_rows + (kSeedPts + W + 1) * S;
+  for (int rr = threadIdx.x; rr < nrows; rr += kSeedPts) {
+    const float4* src = reinterpret_cast<const float4*>(Xs + (sb + rr) * D);
+    float4* dst = reinterpret_cast<float4*>(lds_rows + rr * S);
+    float nrm = 0.0f;
+#pragma unroll
+    for (int k = 0; k < D / 4; ++k) {
+      const float4 x = src[k];
+      dst[k] = x;
+      nrm = __builtin_fmaf(x.x, x.x, nrm);
+      nrm = __builtin_fmaf(x.y, x.y, nrm);
+      nrm = __builtin_fmaf(x.z, x.z, nrm);
+      nrm = __builtin_fmaf(x.w, x.w, nrm);
+    }
+    s_norm[rr] = nrm;
+  }
+  __syncthreads();
+  const int64_t r = r0 + threadIdx.x;
+  if (r >= n) return;  // after the block's only barrier
+  const int64_t lo = seed_lo<W>(r, n);
+  const float* self_row = lds_rows + (r - sb) * S;
+  float4 own[D / 4];
+#pragma unroll
+  for (int k = 0; k < D / 4; ++k) own[k] = reinterpret_cast<const float4*>(self_row)[k];
+  const float own_norm = s_norm[r - sb];
+  uint32_t best[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) best[j] = 0xFFFFFFFFu;
+  for (int slot = 0; slot < W; ++slot) {
+    int64_t p = lo + slot;
+    if (p >= r) p += 1;
+    if (p >= n) break;  // only when n <= W: the remaining slots are past the end too
+    const int pl = static_cast<int>(p - sb);
+    const float4* pr = reinterpret_cast<const float4*>(lds_rows + pl * S);
+    float dot = 0.0f;
+#pragma unroll
+    for (int k = 0; k < D / 4; ++k) {
+      const float4 y = pr[k];
+      dot = __builtin_fmaf(own[k].x, y.x, dot);
+      dot = __builtin_fmaf(own[k].y, y.y, dot);
+      dot = __builtin_fmaf(own[k].z, y.z, dot);
+      dot = __builtin_fmaf(own[k].w, y.w, dot);
+    }
+    const float acc = fmaxf(__builtin_fmaf(-2.0f, dot, own_norm + s_norm[pl]), 0.0f);
+    uint32_t key = (__float_as_uint(acc) & ~static_cast<uint32_t>(W - 1)) | static_cast<uint32_t>(slot);
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const uint32_t lo_k = key < best[j] ? key : best[j];
+      key = key < best[j] ? best[j] : key;
+      best[j] = lo_k;
+    }
+  }
+  uint64_t* out = cand + static_cast<int64_t>(order[r]) * kSeedSlots + slot0;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    uint64_t key = kKeyMax;
+    if (best[j] != 0xFFFFFFFFu) {
+      int64_t p = lo + static_cast<int64_t>(best[j] & static_cast<uint32_t>(W - 1));
+      if (p >= r) p += 1;
+      key = make_key(d2_any(lds_rows + (p - sb) * S, self_row, D), order[p], 1u);
     }
     out[j] = key;
   }
@@ -1074,7 +1158,7 @@ int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* const* nets, int32_t tables, const f
                      int32_t window, uint64_t seed, hdb_nnd_element* lists) {
   if (!ctx || !nets || tables < 1 || tables > kSeedTables || !X || !lists || n < 34)
     return ctx ? ctx->fail(HDB_EINVAL, "bad learned_init args (1..8 networks, n must exceed 33)") : 1;
-  if (window != 128 && window != 256) return ctx->fail(HDB_EINVAL, "window must be 128 or 256");
+  if (window != 128 && window != 256 && window != 512) return ctx->fail(HDB_EINVAL, "window must be 128, 256 or 512");
   for (int t = 0; t < tables; ++t)
     if (!nets[t] || nets[t]->L.kin != d) return ctx->fail(HDB_EINVAL, "network input width differs from d");
   if (n >= (1ll << 31)) return ctx->fail(HDB_EUNSUPPORTED, "more than 2^31 points");
@@ -1132,12 +1216,30 @@ int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* const* nets, int32_t tables, const f
 #define HDB_SEED_W(D_, ST_)                                \
   do {                                                     \
     if (window == 128) HDB_SEED_K(D_, 128, ST_);           \
-    else HDB_SEED_K(D_, 256, ST_);                         \
+    else if (window == 256) HDB_SEED_K(D_, 256, ST_);      \
+    else HDB_SEED_K(D_, 512, ST_);                         \
   } while (0)
-    if (d == 12 && staged) HDB_SEED_W(12, true);
-    else if (d == 32 && staged) HDB_SEED_W(32, true);
+#define HDB_SEED4(D_, W_)                                                                                         \
+  do {                                                                                                            \
+    const size_t l4 = static_cast<size_t>(kSeedPts + W_ + 1) * ((((D_) / 4) & 1) ? (D_) : (D_) + 4) * 4 +       \
+                      static_cast<size_t>(kSeedPts + W_ + 1) * 4;                                                 \
+    if (keep == 16)                                                                                               \
+      hipLaunchKernelGGL((k_mlp_seed4<D_, W_, 16>), grid, dim3(kSeedPts), l4, ctx->stream, Xs, ids, n, slot0, cand); \
+    else                                                                                                          \
+      hipLaunchKernelGGL((k_mlp_seed4<D_, W_, 8>), grid, dim3(kSeedPts), l4, ctx->stream, Xs, ids, n, slot0, cand);  \
+  } while (0)
+#define HDB_SEED4_W(D_)                 \
+  do {                                  \
+    if (window == 128) HDB_SEED4(D_, 128); \
+    else if (window == 256) HDB_SEED4(D_, 256); \
+    else HDB_SEED4(D_, 512);            \
+  } while (0)
+    if (d == 12) HDB_SEED4_W(12);
+    else if (d == 32) HDB_SEED4_W(32);
     else if (staged) HDB_SEED_W(0, true);
     else HDB_SEED_W(0, false);
+#undef HDB_SEED4_W
+#undef HDB_SEED4
 #undef HDB_SEED_W
 #undef HDB_SEED_K
 #undef HDB_SEED

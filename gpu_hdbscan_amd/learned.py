@@ -24,14 +24,14 @@ from ._lib import NND_ELEMENT
 class LearnedInit:
     """`tables` networks (1..8); each network's 12 sign bits are one bucket
     table of the initial lists (see `fit` for the two training losses).
-    `window`: bucket-order positions examined per point and table (128 or
-    256)."""
+    `window`: bucket-order positions examined per point and table (128, 256
+    or 512)."""
 
-    def __init__(self, ctx: api.Context, d: int, seed: int = 0, tables: int = 6, window: int = 256):
+    def __init__(self, ctx: api.Context, d: int, seed: int = 0, tables: int = 8, window: int = 512):
         if not 1 <= tables <= 8:
             raise ValueError("tables must be in 1..8")
-        if window not in (128, 256):
-            raise ValueError("window must be 128 or 256")
+        if window not in (128, 256, 512):
+            raise ValueError("window must be 128, 256 or 512")
         self.ctx = ctx
         self.window = int(window)
         self.d = int(d)

@@ -254,7 +254,7 @@ int hdb_rp_codes(hdb_ctx* ctx, const float* X_dev, int64_t n, int32_t d, uint64_
  * distances; hdb_nndescent_from recomputes them): network t's 12 sign bits
  * are table t's bucket code, points are ordered by (code, x . u_t), and
  * each point keeps the nearest 16 (tables <= 4) or 8 (more tables) of the
- * `window` (128 or 256) points around it in each table; the 32 nearest
+ * `window` (128, 256 or 512) points around it in each table; the 32 nearest
  * distinct ids of all tables form its list. */
 int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* const* nets, int32_t tables, const float* X_dev,
                      int64_t n, int32_t d, int32_t window, uint64_t seed, hdb_nnd_element* lists_dev);

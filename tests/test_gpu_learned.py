@@ -99,7 +99,7 @@ def test_rp_codes_balanced_and_local(ctx):
     assert same_knn > 20 * same_rnd
 
 
-@pytest.mark.parametrize("tables,window,d", [(1, 128, 12), (3, 256, 20), (8, 128, 12), (8, 256, 32)])
+@pytest.mark.parametrize("tables,window,d", [(1, 128, 12), (3, 256, 20), (8, 128, 12), (8, 256, 32), (2, 512, 12)])
 def test_learned_init_lists_valid(ctx, tables, window, d):
     """Initial lists from 1..8 tables and both windows: ids in range, never
     the point itself, distinct per list, sorted, with exact FP32 distances
