@@ -13,9 +13,9 @@ brute force; a leg counts only at recall >= --target):
   plain       random init, NND_ITERATION=6, join_keep 4 (the library
               defaults); its kernels supply stage_ms, kernel_ms and roofline
   plain_best  random init at its fastest (join_keep, iterations)
-  learned     learned init (8 networks trained once on a 200k subsample, the
-              training time reported apart; window 512) at its fastest
-              (join_keep, iterations)
+  learned     learned init (8 networks and window 512 at d <= 16, else 6 and
+              256; trained once on a 200k subsample, the training time
+              reported apart) at its fastest (join_keep, iterations)
 value = points / second of the learned leg (of the plain leg when the
 learned leg is skipped or misses the target).
 """
@@ -50,8 +50,10 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="time the plain (random-init) form only")
     ap.add_argument("--learned-subsample", type=int, default=200000)
-    ap.add_argument("--learned-tables", type=int, default=8, help="bucket tables (networks) of the learned init")
-    ap.add_argument("--learned-window", type=int, default=512, help="bucket-order window per point and table")
+    ap.add_argument("--learned-tables", type=int, default=None,
+                    help="bucket tables (networks) of the learned init (default: by dimension)")
+    ap.add_argument("--learned-window", type=int, default=None,
+                    help="bucket-order window per point and table (default: by dimension)")
     ap.add_argument("--target", type=float, default=0.95, help="recall@32 a timed form must reach")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, three device-to-device copies of this size (counter calibration)")
@@ -197,7 +199,7 @@ def main():
         if lb:
             legs["learned"] = {"join_keep": lb[1], "iterations": lb[2], "ms_per_step": round(lb[0], 3),
                                "points_per_s": n / (lb[0] * 1e-3), "recall_at_32": round(lb[3], 4),
-                               "tables": a.learned_tables, "window": a.learned_window,
+                               "tables": len(model.handles), "window": model.window,
                                "train_s": round(model.train_seconds, 3),
                                "train_subsample": min(a.learned_subsample, n),
                                "final_loss": round(float(model.final_loss), 5),
