@@ -137,6 +137,27 @@ __device__ __forceinline__ float d2_fixed(const float* __restrict__ a, const flo
   return acc;
 }
 
+// Median of three unsigned values in one instruction.
+__device__ __forceinline__ uint32_t med3_u32(uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t r;
+  asm("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
+// Insert key into the ascending list top[0..K) and drop the largest: slot 0
+// takes min(top[0], key) and slot j the median of top[j-1], key and top[j]
+// (the key clamped to [top[j-1], top[j]]); every slot reads the old list, so
+// the K operations are independent.
+template <int K>
+__device__ __forceinline__ void top_insert_med3(uint32_t (&top)[K], uint32_t key) {
+  uint32_t nt[K];
+  nt[0] = top[0] < key ? top[0] : key;
+#pragma unroll
+  for (int j = 1; j < K; ++j) nt[j] = med3_u32(top[j - 1], key, top[j]);
+#pragma unroll
+  for (int j = 0; j < K; ++j) top[j] = nt[j];
+}
+
 __device__ __forceinline__ float d2_any(const float* __restrict__ a, const float* __restrict__ b, int d) {
   float acc = 0.0f;
   for (int k = 0; k < d; ++k) {

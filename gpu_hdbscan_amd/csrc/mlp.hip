@@ -652,8 +652,8 @@ __device__ __forceinline__ int64_t seed_lo(int64_t r, int64_t n) {
 
 // Lane r (a position in the table's bucket order) keeps the K nearest of
 // the W positions around it (itself skipped), ranked by 32-bit keys: the
-// distance bits with the low log2(W) replaced by the window slot, kept by a
-// compare-exchange insertion network in registers.  The winners' exact keys
+// distance bits with the low log2(W) replaced by the window slot, kept in
+// registers by a median-of-three insertion.  The winners' exact keys
 // are recomputed.  Rows come from LDS when STAGED (stride d + 1, an odd
 // stride across lanes); D > 0 fixes d and keeps the point's own row in
 // registers.
@@ -707,12 +707,7 @@ __global__ __launch_bounds__(kSeedPts) void k_mlp_seed(const float* __restrict__
       acc = d2_any(pr, self_row, d);
     }
     uint32_t key = (__float_as_uint(acc) & ~static_cast<uint32_t>(W - 1)) | static_cast<uint32_t>(slot);
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-      const uint32_t lo_k = key < best[j] ? key : best[j];
-      key = key < best[j] ? best[j] : key;
-      best[j] = lo_k;
-    }
+    top_insert_med3<K>(best, key);
   }
   uint64_t* out = cand + static_cast<int64_t>(order[r]) * kSeedSlots + slot0;
 #pragma unroll
@@ -791,12 +786,7 @@ __global__ __launch_bounds__(kSeedPts) void k_mlp_seed4(const float* __restrict_
     }
     const float acc = fmaxf(__builtin_fmaf(-2.0f, dot, own_norm + s_norm[pl]), 0.0f);
     uint32_t key = (__float_as_uint(acc) & ~static_cast<uint32_t>(W - 1)) | static_cast<uint32_t>(slot);
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-      const uint32_t lo_k = key < best[j] ? key : best[j];
-      key = key < best[j] ? best[j] : key;
-      best[j] = lo_k;
-    }
+    top_insert_med3<K>(best, key);
   }
   uint64_t* out = cand + static_cast<int64_t>(order[r]) * kSeedSlots + slot0;
 #pragma unroll

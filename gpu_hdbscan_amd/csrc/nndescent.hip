@@ -529,12 +529,7 @@ __device__ __forceinline__ uint32_t order_bits(float x) {
 
 template <int SLOTS>
 __device__ __forceinline__ void top_insert32(uint32_t (&top)[SLOTS], uint32_t key) {
-#pragma unroll
-  for (int s = 0; s < SLOTS; ++s) {
-    const uint32_t a = top[s];
-    top[s] = a < key ? a : key;
-    key = a < key ? key : a;
-  }
+  top_insert_med3<SLOTS>(top, key);
 }
 
 // Partner-split FP32 join for D <= 32.  Phase 1: every candidate ranks each
