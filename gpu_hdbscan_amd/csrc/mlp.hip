@@ -478,8 +478,8 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
   p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
 }
 
-// Sort keys of one bucket table: (12 sign bits of the embedding z << 20) |
-// 20-bit rank of x.u (u a fixed unit direction, x.u clamped to [-lim, lim]).
+// Sort keys of one bucket table: (12 sign bits of the embedding z << 12) |
+// 12-bit rank of x.u (u a fixed unit direction, x.u clamped to [-lim, lim]).
 __global__ void k_mlp_keys(const float* __restrict__ Z, const float* __restrict__ X, int d,
                            const float* __restrict__ u, float lim, int64_t n, uint32_t* __restrict__ keys,
                            uint32_t* __restrict__ ids) {
@@ -491,8 +491,8 @@ __global__ void k_mlp_keys(const float* __restrict__ Z, const float* __restrict_
   for (int k = 0; k < d; ++k) s = __builtin_fmaf(X[i * d + k], u[k], s);
   float t = (s / lim) * 0.5f + 0.5f;
   t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
-  const uint32_t q = static_cast<uint32_t>(t * 1048575.0f);
-  keys[i] = (code << 20) | q;
+  const uint32_t q = static_cast<uint32_t>(t * 4095.0f);  // 12 bits: a 24-bit key sorts in three passes
+  keys[i] = (code << 12) | q;
   ids[i] = static_cast<uint32_t>(i);
 }
 
@@ -1191,7 +1191,7 @@ int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* const* nets, int32_t tables, const f
     hipLaunchKernelGGL(k_mlp_keys, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, Z, X, d, dirs + t * d, 64.0f,
                        n, keys, ids);
     HDB_LAUNCHED();
-    HDB_CALL(radix_sort_u32_u32(ctx, keys, ids, n, 0, 32));
+    HDB_CALL(radix_sort_u32_u32(ctx, keys, ids, n, 0, 24));
     hipLaunchKernelGGL(k_mlp_permute, dim3(grid_for(n * d, 256)), dim3(256), 0, ctx->stream, X, ids, n, d, Xs);
     HDB_LAUNCHED();
     const int slot0 = t * keep;
