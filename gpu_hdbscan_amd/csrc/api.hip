@@ -99,6 +99,18 @@ int hdb_ctx_create(hdb_ctx** out, int device) {
     delete ctx;
     return HDB_EDEVICE;
   }
+  void* pin = nullptr;
+  if (hipHostMalloc(&pin, kPinnedWords * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->ready[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->ready[1], hipEventDisableTiming) != hipSuccess) {
+    if (pin) (void)hipHostFree(pin);
+    if (ctx->ready[0]) (void)hipEventDestroy(ctx->ready[0]);
+    if (ctx->ready[1]) (void)hipEventDestroy(ctx->ready[1]);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return HDB_EDEVICE;
+  }
+  ctx->pinned = static_cast<int64_t*>(pin);
   *out = ctx;
   return HDB_OK;
 }
@@ -113,6 +125,9 @@ int hdb_ctx_destroy(hdb_ctx* ctx) {
   }
   for (int s = 0; s < kSlotCount; ++s)
     if (ctx->slot_ptr[s]) (void)hipFree(ctx->slot_ptr[s]);
+  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+  for (hipEvent_t e : ctx->ready)
+    if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return HDB_OK;

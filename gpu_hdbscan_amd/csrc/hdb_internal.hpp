@@ -59,6 +59,9 @@ struct hdb_ctx {
   std::map<std::string, hdb::TimerAcc> timers;
   std::vector<std::string> timer_names;  // stable storage for hdb_timer_read
   int64_t stats_host[4] = {0, 0, 0, 0};
+  // Pinned host words and events for readbacks that overlap later work.
+  int64_t* pinned = nullptr;  // kPinnedWords, from hipHostMalloc
+  hipEvent_t ready[2] = {nullptr, nullptr};
   // Pipeline CSR (kept when requested, for tests).
   int64_t csr_edges = 0;
 
@@ -342,6 +345,12 @@ int radix_sort_u32_u64_segs(hdb_ctx* ctx, uint32_t* keys, uint64_t* vals, int64_
 int radix_sort_u32_u32_out(hdb_ctx* ctx, uint32_t* keys, uint32_t* vals, int64_t n, int bit0, int bit1,
                            uint32_t** kout, uint32_t** vout);
 int read_i64(hdb_ctx* ctx, const int64_t* p, int64_t* out);
+// Queue a copy of *p into pinned word `slot` (0 or 1) and mark it with
+// event `slot`; wait_i64 waits for that event only, so kernels queued after
+// the copy keep running.
+constexpr int kPinnedWords = 8;
+int read_i64_async(hdb_ctx* ctx, const int64_t* p, int slot);
+int wait_i64(hdb_ctx* ctx, int slot, int64_t* out);
 int read_i32(hdb_ctx* ctx, const int32_t* p, int32_t* out);
 
 // nndescent.hip

@@ -330,19 +330,22 @@ int mst_csr(hdb_ctx* ctx, const int64_t* nindex, const int32_t* nlist, const flo
   hipLaunchKernelGGL(k_iota, dim3(gv), dim3(256), 0, ctx->stream, next, n);
   HDB_LAUNCHED();
   HDB_CALL(fill_i32(ctx, reinterpret_cast<int32_t*>(best_w), n, 0));  // every vertex starts live
-  HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(cnt), 2, 0));    // cnt[1]: recorded edges
+  HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(cnt), 4, 0));    // cnt[1]: recorded edges
   const int64_t cap = n - 1;
+  // Round r raises flag cnt[2 + r % 2] when some vertex still has an
+  // external edge, and its value is copied to pinned host memory behind the
+  // round.  The host waits for round r's copy only after queueing round
+  // r + 1, so the GPU does not wait on the host; a round after the last
+  // active one finds every vertex dead and changes nothing.
   for (int round = 0;; ++round) {
-    if (round > 64) return ctx->fail(HDB_EDEVICE, "Boruvka did not converge");
+    if (round > 65) return ctx->fail(HDB_EDEVICE, "Boruvka did not converge");
+    unsigned long long* flag = cnt + 2 + (round & 1);
     HDB_CALL(fill_i32(ctx, reinterpret_cast<int32_t*>(cw), n, static_cast<int32_t>(kNoEdge)));
     HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(cp), n, ~0ull));
-    HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(cnt), 1, 0));
+    HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(flag), 1, 0));
     hipLaunchKernelGGL(k_bv_best, dim3(gs), dim3(256), 0, ctx->stream, nindex, nlist, w, n, comp, best_w, best_p, cw,
-                       cnt);
+                       flag);
     HDB_LAUNCHED();
-    int64_t active = 0;
-    HDB_CALL(read_i64(ctx, reinterpret_cast<const int64_t*>(cnt), &active));
-    if (active == 0) break;
     hipLaunchKernelGGL(k_bv_pair, dim3(gv), dim3(256), 0, ctx->stream, n, comp, best_w, best_p, cw, cp);
     HDB_LAUNCHED();
     hipLaunchKernelGGL(k_bv_hook, dim3(gh), dim3(256), 0, ctx->stream, n, comp, best_w, best_p, cw, cp, next, out, cap,
@@ -350,6 +353,11 @@ int mst_csr(hdb_ctx* ctx, const int64_t* nindex, const int32_t* nlist, const flo
     HDB_LAUNCHED();
     hipLaunchKernelGGL(k_bv_jump, dim3(gv), dim3(256), 0, ctx->stream, n, comp, next);
     HDB_LAUNCHED();
+    HDB_CALL(read_i64_async(ctx, reinterpret_cast<const int64_t*>(flag), round & 1));
+    if (round == 0) continue;
+    int64_t active = 0;
+    HDB_CALL(wait_i64(ctx, (round - 1) & 1, &active));
+    if (active == 0) break;
   }
   int64_t m = 0;
   HDB_CALL(read_i64(ctx, reinterpret_cast<const int64_t*>(cnt + 1), &m));

@@ -30,6 +30,18 @@ int read_i64(hdb_ctx* ctx, const int64_t* p, int64_t* out) {
   HDB_HIP(hipStreamSynchronize(ctx->stream));
   return HDB_OK;
 }
+int read_i64_async(hdb_ctx* ctx, const int64_t* p, int slot) {
+  if (!ctx->pinned || slot < 0 || slot > 1) return ctx->fail(HDB_EINVAL, "bad readback slot");
+  HDB_HIP(hipMemcpyAsync(ctx->pinned + slot, p, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+  HDB_HIP(hipEventRecord(ctx->ready[slot], ctx->stream));
+  return HDB_OK;
+}
+int wait_i64(hdb_ctx* ctx, int slot, int64_t* out) {
+  if (!ctx->pinned || slot < 0 || slot > 1) return ctx->fail(HDB_EINVAL, "bad readback slot");
+  HDB_HIP(hipEventSynchronize(ctx->ready[slot]));
+  *out = ctx->pinned[slot];
+  return HDB_OK;
+}
 int read_i32(hdb_ctx* ctx, const int32_t* p, int32_t* out) {
   HDB_HIP(hipMemcpyAsync(out, p, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
   HDB_HIP(hipStreamSynchronize(ctx->stream));
