@@ -123,3 +123,22 @@ def test_replay_same_seed_inputs(ctx):
     else:
         # drops depend on atomic arrival order; sets still agree closely
         assert O.recall(a["label"], b["label"], 32) > 0.99
+
+
+@pytest.mark.parametrize("keep", [0, 3])
+def test_list_distances_never_grow(ctx, keep):
+    """An update only replaces entries by closer ones, so after every
+    iteration (from the second on) each list's k-th distance, and the sum
+    over its entries, is at most what it was.  Runs of 0..4 iterations are
+    prefixes of one another: the path is deterministic."""
+    print(f"case: per-iteration list monotonicity 20000x12 keep={keep}", flush=True)
+    X = datasets.tiny_blobs(20000, 12, 10, seed=53)
+    prev = None
+    for its in range(0, 5):
+        p = api.default_params(nnd_iteration=its, join_keep=keep)
+        L = api.NNDescent(ctx, X, iteration=its, params=p)
+        d = L["distance"].astype(np.float64)
+        if prev is not None and its >= 2:  # the first update also removes repeated random ids
+            assert np.all(d[:, -1] <= prev[:, -1])
+            assert np.all(d.sum(1) <= prev.sum(1) * (1 + 1e-12))
+        prev = d
