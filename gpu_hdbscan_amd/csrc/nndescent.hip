@@ -551,14 +551,18 @@ __global__ __launch_bounds__(kJoinWaves * 64) void k_nnd_join2(const float* __re
   constexpr int kSlots = KEEP == 0 ? 2 : KEEP;
   constexpr int kBlock = 32 * 33;  // max of nn * (no + 1) over nn + no <= 64
   constexpr uint32_t kNone = 0xFFFFFFFFu;
+  // LDS kept under 32 KB per block (five blocks per CU): norms travel by
+  // readlane, and the old-candidate scratch of join_candidates borrows the
+  // new x old block, which is first written after that function's last
+  // barrier.
   __shared__ float s_vec[WPB][64 * kStride];
   __shared__ float s_blk[WPB][kBlock];
-  __shared__ float s_norm[WPB][64];
   __shared__ uint32_t s_new[WPB][64];
-  __shared__ uint32_t s_old[WPB][64];
   __shared__ uint32_t s_cand[WPB][64];
   __shared__ int32_t s_wcnt[WPB];
   __shared__ unsigned long long s_base;
+  static_assert(kBlock >= 64, "old-candidate scratch fits in a wave's block");
+  uint32_t(*s_old)[64] = reinterpret_cast<uint32_t(*)[64]>(&s_blk[0][0]);
   (void)d;
   const unsigned lane = __lane_id();
   const unsigned w = threadIdx.x >> 6;
@@ -588,7 +592,6 @@ __global__ __launch_bounds__(kJoinWaves * 64) void k_nnd_join2(const float* __re
 #pragma unroll
     for (int k = 0; k < kStride; ++k) s_vec[w][lane * kStride + k] = own[k];
   }
-  s_norm[w][lane] = own_norm;
   __syncthreads();
 
   const int bst = no + 1;  // row stride of the new x old block
@@ -603,7 +606,8 @@ __global__ __launch_bounds__(kJoinWaves * 64) void k_nnd_join2(const float* __re
     for (int k = 0; k < D; ++k) dot = __builtin_fmaf(own[k], row[k], dot);
     // clamped at 0 (cancellation can dip below): non-negative float bits
     // order like the values, so the key needs no sign fix-up
-    const float acc = fmaxf(__builtin_fmaf(-2.0f, dot, own_norm + s_norm[w][j]), 0.0f);
+    const float nj = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(own_norm), j));  // j is uniform
+    const float acc = fmaxf(__builtin_fmaf(-2.0f, dot, own_norm + nj), 0.0f);
     if (active && !is_new) s_blk[w][j * bst + (static_cast<int>(lane) - nn)] = acc;
     const bool ok = active && j != static_cast<int>(lane) && (NEWNEW || !is_new);
     const uint32_t key = ok ? ((__float_as_uint(acc) & ~63u) | static_cast<uint32_t>(j)) : kNone;
