@@ -532,7 +532,7 @@ __global__ void k_mlp_pack(const float* __restrict__ prm, MlpLayout L, __bf16* _
 // Z = forward(X) for R rows: the block keeps the packed weights and biases in
 // LDS and each wave strides over 16-row tiles, with activations stored as
 // BF16 (the value they take as the next layer's MFMA operand).
-__global__ __launch_bounds__(512) void k_mlp_infer(const float* __restrict__ X, int d, int64_t R,
+__global__ __launch_bounds__(1024) void k_mlp_infer(const float* __restrict__ X, int d, int64_t R,
                                                    const __bf16* __restrict__ packed, const float* __restrict__ prm,
                                                    MlpLayout L, float* __restrict__ Z) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_b[];
@@ -550,9 +550,11 @@ __global__ __launch_bounds__(512) void k_mlp_infer(const float* __restrict__ X, 
   const unsigned l = __lane_id();
   const int w = static_cast<int>(threadIdx.x >> 6);
   const int ldi = L.kinp + 8, lda = kHid + 8;
-  __bf16* in = act + static_cast<int64_t>(w) * (16 * ldi + 2 * 16 * lda);
+  // one 16-row activation tile per wave: layer 2 reads all of a1 into its
+  // accumulators before writing its output back over it
+  __bf16* in = act + static_cast<int64_t>(w) * (16 * ldi + 16 * lda);
   __bf16* a1 = in + 16 * ldi;
-  __bf16* a2 = a1 + 16 * lda;
+  __bf16* a2 = a1;
   const int r16 = static_cast<int>(l & 15), kq = static_cast<int>(l >> 4);
   const int64_t ntiles = (R + 15) / 16;
   for (int64_t tile = static_cast<int64_t>(blockIdx.x) * waves + w; tile < ntiles;
@@ -598,6 +600,9 @@ __global__ __launch_bounds__(512) void k_mlp_infer(const float* __restrict__ X, 
         for (int nt = 0; nt < 8; ++nt)
           acc[nt] = mfma(a, *reinterpret_cast<const bf16x8*>(sw + ((ks1 * 8 + nt * 4 + ks) * 64 + l) * 8), acc[nt]);
       }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
       for (int nt = 0; nt < 8; ++nt)
 #pragma unroll
@@ -899,9 +904,9 @@ int mlp_infer(hdb_ctx* ctx, hdb_mlp* net, const float* X, int d, int64_t R, floa
   hipLaunchKernelGGL(k_mlp_pack, dim3(grid_for(static_cast<int64_t>(nfrag) * 64, 256)), dim3(256), 0, ctx->stream,
                      net->prm, L, packed);
   HDB_LAUNCHED();
-  const int waves = L.kinp <= 64 ? 8 : 4;
+  const int waves = L.kinp <= 64 ? 16 : 8;
   const size_t lds = static_cast<size_t>(nfrag) * 1024 + 272 * sizeof(float) +
-                     static_cast<size_t>(waves) * (16 * (L.kinp + 8) + 2 * 16 * (kHid + 8)) * 2;
+                     static_cast<size_t>(waves) * (16 * (L.kinp + 8) + 16 * (kHid + 8)) * 2;
   const int64_t tiles = (R + 15) / 16;
   const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>((tiles + waves - 1) / waves, 512)));
   hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(waves * 64), lds, ctx->stream, X, d, R, packed, net->prm, L, Z);
