@@ -237,12 +237,12 @@ __global__ __launch_bounds__(kRsBlock) void k_rs_scatter(const K* __restrict__ k
                                                          int shift, int64_t ntiles,
                                                          const int64_t* __restrict__ hoff, RsSegs sg) {
   static_assert(kRsBlock == kRsBins, "one thread per digit");
-  __shared__ int32_t wcnt[kRsBlock / 64][kRsBins];
+  // per-wave digit counts fit 16 bits (at most 64 * kRsChunks keys a wave)
+  __shared__ uint16_t wcnt[kRsBlock / 64][kRsBins];
   __shared__ int64_t goff[kRsBins];
   __shared__ int32_t dstart[kRsBins];
   __shared__ int32_t wsum[kRsBlock / 64];
   __shared__ int64_t s_pre[kRsMaxSegs + 1];
-  __shared__ uint8_t dig_of[HAS_V ? kRsTile : 1];
   // staging: keys first, then values, through one buffer
   __shared__ uint64_t s_stage[kRsTile * (sizeof(K) > sizeof(V) || !HAS_V ? sizeof(K) : sizeof(V)) / 8];
   K* skey = reinterpret_cast<K*>(s_stage);
@@ -293,7 +293,7 @@ __global__ __launch_bounds__(kRsBlock) void k_rs_scatter(const K* __restrict__ k
     int32_t base = 0;
     if (act && static_cast<int>(lane) == leader) {
       base = wcnt[w][dg];
-      wcnt[w][dg] = base + __popcll(m);
+      wcnt[w][dg] = static_cast<uint16_t>(base + __popcll(m));
     }
     base = __shfl(base, leader);
     rank[c] = act ? base + __popcll(m & lt) : 0;
@@ -306,7 +306,7 @@ __global__ __launch_bounds__(kRsBlock) void k_rs_scatter(const K* __restrict__ k
 #pragma unroll
   for (int ww = 0; ww < kRsBlock / 64; ++ww) {
     const int32_t c = wcnt[ww][dgt];
-    wcnt[ww][dgt] = tot;
+    wcnt[ww][dgt] = static_cast<uint16_t>(tot);
     tot += c;
   }
   int32_t incl = tot;
@@ -333,11 +333,19 @@ __global__ __launch_bounds__(kRsBlock) void k_rs_scatter(const K* __restrict__ k
   __syncthreads();
   const int64_t tbase = static_cast<int64_t>(blockIdx.x) * kRsTile;
   const int32_t cnt = static_cast<int32_t>(n - tbase < kRsTile ? n - tbase : kRsTile);
-  for (int32_t i = static_cast<int32_t>(threadIdx.x); i < cnt; i += kRsBlock) {
-    const K k = skey[i];
-    const unsigned dd = static_cast<unsigned>(k >> shift) & 0xFFu;
-    kout[goff[dd] + (i - dstart[dd])] = k;
-    dig_of[i] = static_cast<uint8_t>(dd);
+  // each thread keeps the digits of the staged positions it writes
+  // (threadIdx.x + 256 t), for the values' pass
+  uint8_t dpos[kRsChunks];
+#pragma unroll
+  for (int t = 0; t < kRsChunks; ++t) {
+    const int32_t i = static_cast<int32_t>(threadIdx.x) + t * kRsBlock;
+    dpos[t] = 0;
+    if (i < cnt) {
+      const K k = skey[i];
+      const unsigned dd = static_cast<unsigned>(k >> shift) & 0xFFu;
+      kout[goff[dd] + (i - dstart[dd])] = k;
+      dpos[t] = static_cast<uint8_t>(dd);
+    }
   }
   if (HAS_V) {
     __syncthreads();
@@ -345,9 +353,13 @@ __global__ __launch_bounds__(kRsBlock) void k_rs_scatter(const K* __restrict__ k
     for (int c = 0; c < kRsChunks; ++c)
       if (lp[c] >= 0) sval[lp[c]] = val[c];
     __syncthreads();
-    for (int32_t i = static_cast<int32_t>(threadIdx.x); i < cnt; i += kRsBlock) {
-      const unsigned dd = dig_of[i];
-      vout[goff[dd] + (i - dstart[dd])] = sval[i];
+#pragma unroll
+    for (int t = 0; t < kRsChunks; ++t) {
+      const int32_t i = static_cast<int32_t>(threadIdx.x) + t * kRsBlock;
+      if (i < cnt) {
+        const unsigned dd = dpos[t];
+        vout[goff[dd] + (i - dstart[dd])] = sval[i];
+      }
     }
   }
 }
