@@ -140,25 +140,28 @@ __device__ __forceinline__ float d2_fixed(const float* __restrict__ a, const flo
   return acc;
 }
 
-// Median of three unsigned values in one instruction.
-__device__ __forceinline__ uint32_t med3_u32(uint32_t a, uint32_t b, uint32_t c) {
-  uint32_t r;
-  asm("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
-  return r;
+// Bits of max(x, 0) for a ranking key: a negative float (sign bit set) is a
+// negative int, so one integer max clamps it, -0 included, to +0.
+__device__ __forceinline__ uint32_t nonneg_bits(float x) {
+  const int32_t b = __float_as_int(x);
+  return static_cast<uint32_t>(b > 0 ? b : 0);
+}
+
+// top[j] = median of (lo, key, top[j]) in place, in one instruction.
+__device__ __forceinline__ void med3_into_u32(uint32_t& top_j, uint32_t lo, uint32_t key) {
+  asm("v_med3_u32 %0, %1, %2, %0" : "+v"(top_j) : "v"(lo), "v"(key));
 }
 
 // Insert key into the ascending list top[0..K) and drop the largest: slot 0
 // takes min(top[0], key) and slot j the median of top[j-1], key and top[j]
-// (the key clamped to [top[j-1], top[j]]); every slot reads the old list, so
-// the K operations are independent.
+// (the key clamped to [top[j-1], top[j]]).  Slots are rewritten from the
+// last down, so each reads the old top[j-1], and each result stays in its
+// own register (no copies in a rolled loop).
 template <int K>
 __device__ __forceinline__ void top_insert_med3(uint32_t (&top)[K], uint32_t key) {
-  uint32_t nt[K];
-  nt[0] = top[0] < key ? top[0] : key;
 #pragma unroll
-  for (int j = 1; j < K; ++j) nt[j] = med3_u32(top[j - 1], key, top[j]);
-#pragma unroll
-  for (int j = 0; j < K; ++j) top[j] = nt[j];
+  for (int j = K - 1; j >= 1; --j) med3_into_u32(top[j], top[j - 1], key);
+  top[0] = top[0] < key ? top[0] : key;
 }
 
 __device__ __forceinline__ float d2_any(const float* __restrict__ a, const float* __restrict__ b, int d) {
@@ -189,12 +192,30 @@ __device__ __forceinline__ float d2_vec4(const float* __restrict__ a, const floa
   return acc;
 }
 
-// Euclidean distance accumulated in FP64 (no contraction), rounded once.
+// Euclidean distance accumulated in FP64, rounded once: diff = a - b and
+// acc = acc + diff * diff with the product rounded before the add (no fused
+// multiply-add, as the oracle's C with -ffp-contract=off).  Loads are issued
+// four dimensions at a time ahead of the arithmetic.
 __device__ __forceinline__ float dist_fp64(const float* __restrict__ a, const float* __restrict__ b, int d) {
+#pragma clang fp contract(off)
   double acc = 0.0;
-  for (int k = 0; k < d; ++k) {
-    double diff = __dsub_rn(static_cast<double>(a[k]), static_cast<double>(b[k]));
-    acc = __dadd_rn(acc, __dmul_rn(diff, diff));
+  int k = 0;
+  for (; k + 4 <= d; k += 4) {
+    float x[4], y[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      x[t] = a[k + t];
+      y[t] = b[k + t];
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const double diff = static_cast<double>(x[t]) - static_cast<double>(y[t]);
+      acc = acc + diff * diff;
+    }
+  }
+  for (; k < d; ++k) {
+    const double diff = static_cast<double>(a[k]) - static_cast<double>(b[k]);
+    acc = acc + diff * diff;
   }
   return static_cast<float>(__dsqrt_rn(acc));
 }

@@ -806,6 +806,179 @@ __global__ __launch_bounds__(kSeedPts) void k_mlp_seed4(const float* __restrict_
   }
 }
 
+// The window search for d = 12 on the matrix cores.  A wave takes 16
+// consecutive positions and the span of W + 32 rows that holds all their
+// windows (u0 = min(seed_lo(first), n - span)); tile t of the span is 16 rows.
+// Rows are staged with two extra features, so that one 16x16x4 FP32 MFMA
+// chain over k = 16 gives every ranking distance of the tile directly:
+//   A row w = (w_0..w_11, |w|^2, 1, 0, 0), B column p = (-2 p_0..-2 p_11, 1, |p|^2, 0, 0),
+//   (A B)[w][p] = |w|^2 + |p|^2 - 2 w.p.
+// Feature 4s + g of a row is stored at float 4g + s (row stride 20 floats),
+// so lane group g (lanes 16g..16g+15) reads its four k-slices as one
+// float4; 20 floats is an odd number of float4s, so 8 consecutive rows fall
+// in distinct 4-bank groups.  Lane (g, c) receives rows 4g..4g+3 of each
+// tile for position c and keeps the K/2 nearest of them in registers
+// (median-of-three insertion); the four lanes of a position then merge
+// their lists (xor 16, xor 32) into the K + 1 nearest of the union.  A
+// lane's quarter holds more than K/2 of the span's K + 1 nearest rarely
+// (then a slightly farther row takes the place): these are initial
+// candidates, and every key written is exact.  The position itself
+// (distance 0) is dropped when the exact keys are written; each of the four
+// lanes writes every fourth winner.  The span rows beyond a window are
+// further candidates.
+__device__ __forceinline__ float f4_at(const float4& v, int i) {
+  return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
+}
+
+constexpr int kSmWaves = 16;            // waves per block of k_mlp_seed_mfma
+constexpr int kSmPts = 16 * kSmWaves;   // positions per block
+constexpr int kSmStride = 20;           // staged row stride, floats
+
+constexpr int seed_mfma_span(int W) { return 16 * (W / 16 + 2); }
+constexpr size_t seed_mfma_lds(int W) {
+  return static_cast<size_t>(kSmPts - 16 + seed_mfma_span(W)) * kSmStride * sizeof(float);
+}
+
+template <int W, int K>
+__global__ __launch_bounds__(64 * kSmWaves) void k_mlp_seed_mfma(const float* __restrict__ Xs,
+                                                                  const uint32_t* __restrict__ order, int64_t n,
+                                                                  int slot0, uint64_t* __restrict__ cand) {
+  constexpr int D = 12;
+  constexpr int kTiles = W / 16 + 2;
+  constexpr int kSpan = 16 * kTiles;
+  constexpr int kRow4 = kSmStride / 4;  // float4s per staged row
+  constexpr uint32_t kMask = 2u * W - 1u;
+  constexpr int kLane = K / 2;  // nearest kept per lane (a quarter of the span)
+  static_assert((W & (W - 1)) == 0 && W >= 64 && W <= 512, "window is a power of two");
+  static_assert(kSpan <= 2 * W, "span slots fit the key's low bits");
+  static_assert(4 * kLane >= K + 1 && 2 * kLane <= K + 1, "lane lists cover the K + 1 nearest");
+  extern __shared__ __attribute__((aligned(16))) float4 lds_span4[];
+  auto span_lo = [n](int64_t r) {
+    const int64_t lo = seed_lo<W>(r, n);
+    return lo < n - kSpan ? lo : n - kSpan;
+  };
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kSmPts;
+  const int64_t sb = span_lo(r0);
+  const int64_t last_wave = r0 + ((n - r0 < kSmPts ? n - r0 : kSmPts) - 1) / 16 * 16;
+  const int nrows = static_cast<int>(span_lo(last_wave) + kSpan - sb);
+  for (int j = threadIdx.x; j < nrows; j += 64 * kSmWaves) {
+    const float4* src = reinterpret_cast<const float4*>(Xs + (sb + j) * D);
+    const float4 x0 = src[0], x1 = src[1], x2 = src[2];
+    float nrm = 0.0f;
+    nrm = __builtin_fmaf(x0.x, x0.x, nrm);
+    nrm = __builtin_fmaf(x0.y, x0.y, nrm);
+    nrm = __builtin_fmaf(x0.z, x0.z, nrm);
+    nrm = __builtin_fmaf(x0.w, x0.w, nrm);
+    nrm = __builtin_fmaf(x1.x, x1.x, nrm);
+    nrm = __builtin_fmaf(x1.y, x1.y, nrm);
+    nrm = __builtin_fmaf(x1.z, x1.z, nrm);
+    nrm = __builtin_fmaf(x1.w, x1.w, nrm);
+    nrm = __builtin_fmaf(x2.x, x2.x, nrm);
+    nrm = __builtin_fmaf(x2.y, x2.y, nrm);
+    nrm = __builtin_fmaf(x2.z, x2.z, nrm);
+    nrm = __builtin_fmaf(x2.w, x2.w, nrm);
+    float4* dst = lds_span4 + j * kRow4;
+    dst[0] = make_float4(x0.x, x1.x, x2.x, nrm);   // features 0, 4, 8, 12
+    dst[1] = make_float4(x0.y, x1.y, x2.y, 1.0f);  // features 1, 5, 9, 13
+    dst[2] = make_float4(x0.z, x1.z, x2.z, 0.0f);  // features 2, 6, 10, 14
+    dst[3] = make_float4(x0.w, x1.w, x2.w, 0.0f);  // features 3, 7, 11, 15
+  }
+  __syncthreads();
+  const int wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 4;
+  const int c = lane & 15;
+  const int64_t rw = r0 + 16 * wv;
+  if (rw >= n) return;  // wave-uniform, after the block's only barrier
+  const int j0 = static_cast<int>(span_lo(rw) - sb);
+  const int64_t r = rw + c;
+  const bool valid = r < n;
+  const int jr = valid ? static_cast<int>(r - sb) : j0;
+  // B column: this lane's position, k-slices 4s + g scaled by -2, and
+  // features 12..15 = (1, |p|^2, 0, 0)
+  float4 bq = lds_span4[jr * kRow4 + g];
+  const float nr = lds_span4[jr * kRow4].w;
+  bq.x *= -2.0f;
+  bq.y *= -2.0f;
+  bq.z *= -2.0f;
+  bq.w = g == 0 ? 1.0f : (g == 1 ? nr : 0.0f);
+  if (!valid) bq = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  uint32_t part[kLane];
+#pragma unroll
+  for (int j = 0; j < kLane; ++j) part[j] = 0xFFFFFFFFu;
+  const float4* arow = lds_span4 + (j0 + c) * kRow4 + g;
+#pragma unroll 2
+  for (int t = 0; t < kTiles; ++t) {
+    const float4 av = arow[t * 16 * kRow4];
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bq.x, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, bq.y, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, bq.z, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, bq.w, acc, 0, 0, 0);
+    const uint32_t s0 = static_cast<uint32_t>(16 * t + 4 * g);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) top_insert_med3<kLane>(part, (nonneg_bits(acc[q]) & ~kMask) | (s0 + q));
+  }
+  // lanes g and g ^ 1: the nearest 2 kLane of half the span, sorted
+  uint32_t top[K + 1];
+#pragma unroll
+  for (int j = 0; j <= K; ++j) top[j] = j < kLane ? part[j] : 0xFFFFFFFFu;
+#pragma unroll
+  for (int j = 0; j < kLane; ++j)
+    top_insert_med3<K + 1>(top, static_cast<uint32_t>(__shfl_xor(static_cast<int>(part[j]), 16)));
+  // with lanes g ^ 2: over ascending lists A (g < 2) and B (g >= 2),
+  // min(A[j], B[K - j]) is the K + 1 smallest of both (in no particular
+  // order); all four lanes form the same array, so the every-fourth split
+  // below writes each winner once
+  const bool hi = (g & 2) != 0;
+  uint32_t mrg[K + 1];
+#pragma unroll
+  for (int j = 0; j <= K; ++j) {
+    const uint32_t mine = hi ? top[K - j] : top[j];
+    const uint32_t o = static_cast<uint32_t>(__shfl_xor(static_cast<int>(mine), 32));
+    mrg[j] = mine < o ? mine : o;
+  }
+#pragma unroll
+  for (int j = 0; j <= K; ++j) top[j] = mrg[j];
+  if (!valid) return;
+  int selfpos = K + 1;
+#pragma unroll
+  for (int j = 0; j <= K; ++j)
+    if (top[j] != 0xFFFFFFFFu && j0 + static_cast<int>(top[j] & kMask) == jr) selfpos = j;
+  uint64_t* out = cand + static_cast<int64_t>(order[r]) * kSeedSlots + slot0;
+  // feature k of a staged row is float k / 4 of its float4 k % 4
+  float4 self4[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) self4[e] = lds_span4[jr * kRow4 + e];
+#pragma unroll
+  for (int jj = 0; jj < (K + 4) / 4; ++jj) {
+    const int j = 4 * jj + g;
+    if (j > K) break;
+    uint32_t key = top[4 * jj];
+#pragma unroll
+    for (int e = 1; e < 4; ++e)
+      if (4 * jj + e <= K && g == e) key = top[4 * jj + e];
+    const int o = j - (selfpos < j ? 1 : 0);
+    if (j == selfpos || o >= K) continue;
+    uint64_t v = kKeyMax;
+    if (key != 0xFFFFFFFFu) {
+      const int js = j0 + static_cast<int>(key & kMask);
+      float4 row4[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) row4[e] = lds_span4[js * kRow4 + e];
+      // the oracle's operation order: diff = row - self, fmaf over k = 0..11
+      float acc = 0.0f;
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        const float diff = f4_at(row4[k % 4], k / 4) - f4_at(self4[k % 4], k / 4);
+        acc = __builtin_fmaf(diff, diff, acc);
+      }
+      v = make_key(acc, order[sb + js], 1u);
+    }
+    out[o] = v;
+  }
+}
+
 // Per point: the 32 nearest distinct ids among its table candidates, sorted
 // with exact keys.  Slots no candidate fills take xorshift ids, stepped +1
 // past the point itself and past ids already in the list (n > 33 ids leave
@@ -1161,11 +1334,11 @@ int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* const* nets, int32_t tables, const f
   Stage st(ctx, "learned_init");
   const int keep = tables <= 4 ? 16 : 8;  // nearest window entries kept per table
   float *Z, *dirs, *Xs;
-  uint32_t *keys, *ids;
+  uint32_t *keys, *order;
   uint64_t* cand;
   HDB_CALL(scratch_t(ctx, kSlotMisc0, n * kOutP, &Z));
   HDB_CALL(scratch_t(ctx, kSlotMisc1, n, &keys));
-  HDB_CALL(scratch_t(ctx, kSlotMisc2, n, &ids));
+  HDB_CALL(scratch_t(ctx, kSlotMisc2, n, &order));
   HDB_CALL(scratch_t(ctx, kSlotBfTmp, static_cast<int64_t>(tables) * d, &dirs));
   HDB_CALL(scratch_t(ctx, kSlotSeedX, n * d, &Xs));
   HDB_CALL(scratch_t(ctx, kSlotSeedCand, n * kSeedSlots, &cand));
@@ -1190,13 +1363,16 @@ int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* const* nets, int32_t tables, const f
   const size_t lds = static_cast<size_t>(kSeedPts + window + 1) * (d + 1) * sizeof(float);
   const bool staged = lds <= 80 * 1024;
   const dim3 grid(static_cast<unsigned>(grid_for(n, kSeedPts)));
+  const dim3 gridm(static_cast<unsigned>(grid_for(n, kSmPts)));
   for (int t = 0; t < tables; ++t) {
     HDB_CALL(mlp_infer(ctx, nets[t], X, d, n, Z));
     // x.u range: 64 covers the spread of a unit direction over typical data
     hipLaunchKernelGGL(k_mlp_keys, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, Z, X, d, dirs + t * d, 64.0f,
-                       n, keys, ids);
+                       n, keys, order);
     HDB_LAUNCHED();
-    HDB_CALL(radix_sort_u32_u32(ctx, keys, ids, n, 0, 24));
+    // the sorted order stays wherever the last pass left it (no copy back)
+    uint32_t *skeys, *ids;
+    HDB_CALL(radix_sort_u32_u32_out(ctx, keys, order, n, 0, 24, &skeys, &ids));
     hipLaunchKernelGGL(k_mlp_permute, dim3(grid_for(n * d, 256)), dim3(256), 0, ctx->stream, X, ids, n, d, Xs);
     HDB_LAUNCHED();
     const int slot0 = t * keep;
@@ -1229,10 +1405,24 @@ int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* const* nets, int32_t tables, const f
     else if (window == 256) HDB_SEED4(D_, 256); \
     else HDB_SEED4(D_, 512);            \
   } while (0)
-    if (d == 12) HDB_SEED4_W(12);
+#define HDB_SEEDM(W_)                                                                                         \
+  do {                                                                                                        \
+    if (keep == 16)                                                                                           \
+      hipLaunchKernelGGL((k_mlp_seed_mfma<W_, 16>), gridm, dim3(64 * kSmWaves), seed_mfma_lds(W_), ctx->stream, \
+                         Xs, ids, n, slot0, cand);                                                            \
+    else                                                                                                      \
+      hipLaunchKernelGGL((k_mlp_seed_mfma<W_, 8>), gridm, dim3(64 * kSmWaves), seed_mfma_lds(W_), ctx->stream,  \
+                         Xs, ids, n, slot0, cand);                                                            \
+  } while (0)
+    if (d == 12 && n >= window + 32) {
+      if (window == 128) HDB_SEEDM(128);
+      else if (window == 256) HDB_SEEDM(256);
+      else HDB_SEEDM(512);
+    } else if (d == 12) HDB_SEED4_W(12);
     else if (d == 32) HDB_SEED4_W(32);
     else if (staged) HDB_SEED_W(0, true);
     else HDB_SEED_W(0, false);
+#undef HDB_SEEDM
 #undef HDB_SEED4_W
 #undef HDB_SEED4
 #undef HDB_SEED_W

@@ -42,7 +42,12 @@ __device__ __forceinline__ int find_compress(int v, int* __restrict__ parent) {
 // per row; the grid strides over groups of sixteen vertices).  A vertex
 // whose row has no such edge stays dead: components only grow.  Offers the
 // weight to the component (cw); a block with such a vertex sets `active`
-// to 1 with a plain store (no same-address atomics, which serialise).
+// to 1 with a plain store (no same-address atomics, which serialise).  In
+// the FIRST round every vertex is its own component, so every entry but a
+// self entry leaves it and no component is looked up.  Row entries are
+// streamed with non-temporal loads, which keeps the randomly gathered
+// component array in the cache.
+template <bool FIRST>
 __global__ __launch_bounds__(256) void k_bv_best(const int64_t* __restrict__ nindex, const int32_t* __restrict__ nlist,
                                                  const float* __restrict__ w, int64_t n, const int32_t* __restrict__ comp,
                                                  uint32_t* __restrict__ best_w, uint64_t* __restrict__ best_p,
@@ -71,7 +76,7 @@ __global__ __launch_bounds__(256) void k_bv_best(const int64_t* __restrict__ nin
     int64_t other = static_cast<int64_t>(pp >> 32) == vc ? static_cast<int64_t>(pp & 0xFFFFFFFFu)
                                                           : static_cast<int64_t>(pp >> 32);
     if (!had || other < 0 || other >= n) other = vc;
-    const int co = comp[other];
+    const int co = FIRST ? rvv : comp[other];
     // four entries per lane per step: their loads, then their components'
     // loads, are issued together
     int32_t u[4] = {-1, -1, -1, -1};
@@ -81,16 +86,20 @@ __global__ __launch_bounds__(256) void k_bv_best(const int64_t* __restrict__ nin
       for (int t = 0; t < 4; ++t) {
         const int64_t jj = e0 + sl + 16 * t;
         const int64_t jc = jj < e1 ? jj : e0;
-        u[t] = jj < e1 ? nlist[jc] : -1;
-        wt[t] = w[jc];
+        u[t] = jj < e1 ? __builtin_nontemporal_load(nlist + jc) : -1;
+        wt[t] = __builtin_nontemporal_load(w + jc);
       }
     }
     int cu[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const bool ok = u[t] >= 0 && static_cast<int64_t>(u[t]) < n;
-      cu[t] = comp[ok ? u[t] : vc];
-      if (!ok) cu[t] = rvv;
+      if (FIRST) {
+        cu[t] = ok && static_cast<int64_t>(u[t]) != vc ? u[t] : rvv;
+      } else {
+        cu[t] = comp[ok ? u[t] : vc];
+        if (!ok) cu[t] = rvv;
+      }
     }
     if (live) rv = rvv;
     if (had && co != rvv) {
@@ -116,14 +125,18 @@ __global__ __launch_bounds__(256) void k_bv_best(const int64_t* __restrict__ nin
         for (int t = 0; t < 4; ++t) {
           const int64_t jj = j + 16 * t;
           const int64_t jc = jj < e1 ? jj : j;
-          u[t] = jj < e1 ? nlist[jc] : -1;
-          wt[t] = w[jc];
+          u[t] = jj < e1 ? __builtin_nontemporal_load(nlist + jc) : -1;
+          wt[t] = __builtin_nontemporal_load(w + jc);
         }
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
           const bool ok = u[t] >= 0 && static_cast<int64_t>(u[t]) < n;
-          cu[t] = comp[ok ? u[t] : vc];
-          if (!ok) cu[t] = rv;
+          if (FIRST) {
+            cu[t] = ok && static_cast<int64_t>(u[t]) != vc ? u[t] : rv;
+          } else {
+            cu[t] = comp[ok ? u[t] : vc];
+            if (!ok) cu[t] = rv;
+          }
         }
       }
     }
@@ -343,8 +356,12 @@ int mst_csr(hdb_ctx* ctx, const int64_t* nindex, const int32_t* nlist, const flo
     HDB_CALL(fill_i32(ctx, reinterpret_cast<int32_t*>(cw), n, static_cast<int32_t>(kNoEdge)));
     HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(cp), n, ~0ull));
     HDB_CALL(fill_u64(ctx, reinterpret_cast<uint64_t*>(flag), 1, 0));
-    hipLaunchKernelGGL(k_bv_best, dim3(gs), dim3(256), 0, ctx->stream, nindex, nlist, w, n, comp, best_w, best_p, cw,
-                       flag);
+    if (round == 0)
+      hipLaunchKernelGGL(k_bv_best<true>, dim3(gs), dim3(256), 0, ctx->stream, nindex, nlist, w, n, comp, best_w, best_p,
+                         cw, flag);
+    else
+      hipLaunchKernelGGL(k_bv_best<false>, dim3(gs), dim3(256), 0, ctx->stream, nindex, nlist, w, n, comp, best_w,
+                         best_p, cw, flag);
     HDB_LAUNCHED();
     hipLaunchKernelGGL(k_bv_pair, dim3(gv), dim3(256), 0, ctx->stream, n, comp, best_w, best_p, cw, cp);
     HDB_LAUNCHED();

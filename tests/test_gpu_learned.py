@@ -99,14 +99,19 @@ def test_rp_codes_balanced_and_local(ctx):
     assert same_knn > 20 * same_rnd
 
 
-@pytest.mark.parametrize("tables,window,d", [(1, 128, 12), (3, 256, 20), (8, 128, 12), (8, 256, 32), (2, 512, 12)])
-def test_learned_init_lists_valid(ctx, tables, window, d):
-    """Initial lists from 1..8 tables and both windows: ids in range, never
+@pytest.mark.parametrize("tables,window,d,n", [(1, 128, 12, 5000), (3, 256, 20, 5000), (8, 128, 12, 5000),
+                                                (8, 256, 32, 5000), (2, 512, 12, 5000), (2, 512, 12, 4097),
+                                                (3, 512, 12, 514), (1, 512, 12, 513), (2, 256, 12, 300),
+                                                (1, 256, 12, 287), (1, 128, 12, 35)])
+def test_learned_init_lists_valid(ctx, tables, window, d, n):
+    """Initial lists from 1..8 tables and every window: ids in range, never
     the point itself, distinct per list, sorted, with exact FP32 distances
-    (the oracle's fused-multiply-add order), random fills included."""
-    print(f"case: learned init lists tables={tables} window={window} d={d}", flush=True)
-    rng = np.random.default_rng(tables * 1000 + window + d)
-    n = 5000
+    (the oracle's fused-multiply-add order), random fills included.  For
+    d = 12 the sizes cover both search forms near their bounds: the matrix
+    core search (n >= window + 32, a last wave with one position at
+    n = 4097) and the lane-per-position search below it, down to n = 35."""
+    print(f"case: learned init lists tables={tables} window={window} d={d} n={n}", flush=True)
+    rng = np.random.default_rng(tables * 1000 + window + d + n)
     X = (rng.standard_normal((n, d)) * rng.uniform(0.5, 3.0, d)).astype(np.float32)
     net = learned.LearnedInit(ctx, d, seed=2, tables=tables, window=window)
     L = net.init_lists(ctx.to_device(X)).numpy()
