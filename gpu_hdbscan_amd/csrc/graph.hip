@@ -69,6 +69,35 @@ __global__ void k_antihub_mark(const hdb_nnd_element* __restrict__ lists, const 
   }
 }
 
+// Up to kAhSmall tied keys: one block sorts them in LDS (bitonic, padded to
+// a power of two with ~0) and flags the first `need` -- one launch in place
+// of a radix sort's forty for a handful of keys.
+constexpr int kAhSmall = 4096;
+__global__ __launch_bounds__(1024) void k_antihub_take_small(const uint64_t* __restrict__ tied, int64_t m,
+                                                             int64_t need, int32_t* __restrict__ flag) {
+  __shared__ uint64_t s[kAhSmall];
+  int p = 1;
+  while (p < m) p <<= 1;
+  for (int i = threadIdx.x; i < p; i += blockDim.x) s[i] = i < m ? tied[i] : ~0ull;
+  __syncthreads();
+  for (int k = 2; k <= p; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < p; i += blockDim.x) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const uint64_t a = s[i], b = s[ixj];
+          if ((a > b) == ((i & k) == 0)) {
+            s[i] = b;
+            s[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < need; i += blockDim.x) flag[static_cast<uint32_t>(s[i])] = 1;
+}
+
 __global__ void k_antihub_take(const uint64_t* __restrict__ sorted, int64_t need, int32_t* __restrict__ flag) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < need) flag[static_cast<uint32_t>(sorted[i])] = 1;
@@ -127,8 +156,12 @@ int antihubs(hdb_ctx* ctx, const hdb_nnd_element* lists, const int32_t* counts, 
                      tied, ntied);
   HDB_LAUNCHED();
   // lowest (score, id) among the tied (calculate_finalAntihubs, calculates.cu:491-523)
-  HDB_CALL(radix_sort_u64(ctx, tied, ntied_h, 0, 64));
-  hipLaunchKernelGGL(k_antihub_take, dim3(grid_for(need, 256)), dim3(256), 0, ctx->stream, tied, need, flag);
+  if (ntied_h <= kAhSmall) {
+    hipLaunchKernelGGL(k_antihub_take_small, dim3(1), dim3(1024), 0, ctx->stream, tied, ntied_h, need, flag);
+  } else {
+    HDB_CALL(radix_sort_u64(ctx, tied, ntied_h, 0, 64));
+    hipLaunchKernelGGL(k_antihub_take, dim3(grid_for(need, 256)), dim3(256), 0, ctx->stream, tied, need, flag);
+  }
   HDB_LAUNCHED();
   int64_t* off;
   HDB_CALL(scratch_t(ctx, kSlotMisc0, n + 1, &off));

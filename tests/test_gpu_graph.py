@@ -32,6 +32,22 @@ def test_antihubs(ctx, graph):
     np.testing.assert_array_equal(api.find_antihubs(ctx, L, c), O.antihubs(L, c))
 
 
+@pytest.mark.parametrize("n", [3000, 20000])
+def test_antihubs_many_ties(ctx, n):
+    """Every list holds ids 0..32 (itself skipped), so all other vertices tie
+    at in-degree 0 with equal scores: the anti-hubs are the lowest ids among
+    them.  n = 3000 sorts the 2,967 tied keys in one block, n = 20000 the
+    19,967 tied keys with the radix sort."""
+    print(f"case: antihubs many ties n={n}", flush=True)
+    L = np.empty((n, 32), api.NND_ELEMENT)
+    for i in range(n):
+        ids = [j for j in range(33) if j != i][:32]
+        L[i]["label"] = ids
+        L[i]["distance"] = np.arange(32, dtype=np.float32)
+    c = O.hub_counts(L)
+    np.testing.assert_array_equal(api.find_antihubs(ctx, L, c), O.antihubs(L, c))
+
+
 def test_core_distances(ctx, graph):
     X, L = graph
     got = api.calculateCoreDistance(ctx, X, L, 32)
@@ -90,6 +106,39 @@ def test_mst_tie_heavy_and_disconnected(ctx):
     got, comps = api.gpuMST(ctx, g)
     ref, rcomps = O.kruskal(ni, nl, w)
     assert comps == rcomps == 2
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_mst_long_rows_and_ties(ctx):
+    """Rows of a few to two thousand entries (many 64-entry steps per row,
+    where each step's lookups are bounded by the row's best so far) with
+    weights from eleven levels, so ties are decided by the vertex pair."""
+    print("case: mst long rows 2000 vertices, 40 hubs", flush=True)
+    rng = np.random.default_rng(11)
+    n, hubs = 2000, 40
+    rows = [set() for _ in range(n)]
+    for i in range(n):
+        for j in rng.choice(n, 5, replace=False):
+            if j != i:
+                rows[i].add(int(j))
+                rows[j].add(i)
+    for hb in range(hubs):
+        for j in range(n):
+            if j != hb:
+                rows[hb].add(j)
+                rows[j].add(hb)
+    ni = np.zeros(n + 1, np.int64)
+    nl = []
+    for i in range(n):
+        nl.extend(sorted(rows[i]))
+        ni[i + 1] = len(nl)
+    nl = np.array(nl, np.int32)
+    src = np.repeat(np.arange(n), np.diff(ni))
+    w = ((np.minimum(src, nl) * 7 + np.maximum(src, nl) * 13) % 11).astype(np.float32) * np.float32(0.25) + 1
+    g = api.ECLgraph(n, len(nl), ni, nl, w)
+    got, comps = api.gpuMST(ctx, g)
+    ref, rcomps = O.kruskal(ni, nl, w)
+    assert comps == rcomps == 1
     np.testing.assert_array_equal(got, ref)
 
 
