@@ -598,21 +598,42 @@ __global__ __launch_bounds__(kJoinWaves * 64) void k_nnd_join2(const float* __re
   uint32_t top[kSlots];
 #pragma unroll
   for (int s = 0; s < kSlots; ++s) top[s] = kNone;
-  // phase 1: partners among the new candidates
-  for (int j = 0; j < nn; ++j) {
+  // phase 1: partners among the new candidates, two per step: both rows'
+  // LDS reads are issued before either dot product, and one branch covers
+  // both stores into the new x old block
+  const bool store_old = active && !is_new;
+  float* blk_col = s_blk[w] + (static_cast<int>(lane) - nn);  // + j * bst
+  auto partner = [&](int j) {
     const float* row = s_vec[w] + j * kStride;
     float dot = 0.0f;
 #pragma unroll
     for (int k = 0; k < D; ++k) dot = __builtin_fmaf(own[k], row[k], dot);
-    // clamped at 0 (cancellation can dip below): non-negative float bits
-    // order like the values, so the key needs no sign fix-up
+    // clamped at 0 (cancellation can dip below) by one integer max on the
+    // bits: non-negative float bits order like the values
     const float nj = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(own_norm), j));  // j is uniform
-    const float acc = fmaxf(__builtin_fmaf(-2.0f, dot, own_norm + nj), 0.0f);
-    if (active && !is_new) s_blk[w][j * bst + (static_cast<int>(lane) - nn)] = acc;
+    return __uint_as_float(nonneg_bits(__builtin_fmaf(-2.0f, dot, own_norm + nj)));
+  };
+  auto rank = [&](int j, float acc) {
     const bool ok = active && j != static_cast<int>(lane) && (NEWNEW || !is_new);
     const uint32_t key = ok ? ((__float_as_uint(acc) & ~63u) | static_cast<uint32_t>(j)) : kNone;
     if (KEEP == 0) top[0] = key < top[0] ? key : top[0];
     else top_insert32<kSlots>(top, key);
+  };
+  int j = 0;
+  for (; j + 2 <= nn; j += 2) {
+    const float a0 = partner(j);
+    const float a1 = partner(j + 1);
+    if (store_old) {
+      blk_col[j * bst] = a0;
+      blk_col[(j + 1) * bst] = a1;
+    }
+    rank(j, a0);
+    rank(j + 1, a1);
+  }
+  if (j < nn) {
+    const float a0 = partner(j);
+    if (store_old) blk_col[j * bst] = a0;
+    rank(j, a0);
   }
   __syncthreads();
   // phase 2: new candidates take their old partners from the block
