@@ -110,9 +110,9 @@ def test_mst_tie_heavy_and_disconnected(ctx):
 
 
 def test_mst_long_rows_and_ties(ctx):
-    """Rows of a few to two thousand entries (many 64-entry steps per row,
-    where each step's lookups are bounded by the row's best so far) with
-    weights from eleven levels, so ties are decided by the vertex pair."""
+    """Rows of a few to two thousand entries (up to 32 steps of 64 entries
+    per row) with weights from eleven levels, so ties are decided by the
+    vertex pair."""
     print("case: mst long rows 2000 vertices, 40 hubs", flush=True)
     rng = np.random.default_rng(11)
     n, hubs = 2000, 40
@@ -140,6 +140,19 @@ def test_mst_long_rows_and_ties(ctx):
     ref, rcomps = O.kruskal(ni, nl, w)
     assert comps == rcomps == 1
     np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("m", [1, 4095, 4097, 1 << 20, (1 << 20) + 4096])
+def test_sort_edges_sizes(ctx, m):
+    """Radix tiles of 4096 keys: one partial tile, one full tile plus one
+    key, and 256 and 257 tiles (histograms of 65,536 and 65,792 entries)."""
+    print(f"case: sort {m} edges", flush=True)
+    rng = np.random.default_rng(m)
+    e = np.empty(m, api.MST_EDGE)
+    e["from_node"] = rng.integers(0, 1 << 24, m)
+    e["to_node"] = rng.integers(0, 1 << 24, m)
+    e["weight"] = rng.integers(0, 1 << 16, m).astype(np.float32) * np.float32(0.01)
+    np.testing.assert_array_equal(api.sort_edges(ctx, e), O.sort_edges(e))
 
 
 def test_sort_edges(ctx):
