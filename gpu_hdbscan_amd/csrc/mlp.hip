@@ -840,7 +840,7 @@ constexpr size_t seed_mfma_lds(int W) {
 }
 
 template <int W, int K>
-__global__ __launch_bounds__(64 * kSmWaves) void k_mlp_seed_mfma(const float* __restrict__ Xs,
+__global__ __launch_bounds__(64 * kSmWaves) void k_mlp_seed_mfma(const float* __restrict__ X,
                                                                   const uint32_t* __restrict__ order, int64_t n,
                                                                   int slot0, uint64_t* __restrict__ cand) {
   constexpr int D = 12;
@@ -862,7 +862,8 @@ __global__ __launch_bounds__(64 * kSmWaves) void k_mlp_seed_mfma(const float* __
   const int64_t last_wave = r0 + ((n - r0 < kSmPts ? n - r0 : kSmPts) - 1) / 16 * 16;
   const int nrows = static_cast<int>(span_lo(last_wave) + kSpan - sb);
   for (int j = threadIdx.x; j < nrows; j += 64 * kSmWaves) {
-    const float4* src = reinterpret_cast<const float4*>(Xs + (sb + j) * D);
+    // row j of the span is point order[sb + j], gathered from X directly
+    const float4* src = reinterpret_cast<const float4*>(X + static_cast<int64_t>(order[sb + j]) * D);
     const float4 x0 = src[0], x1 = src[1], x2 = src[2];
     float nrm = 0.0f;
     nrm = __builtin_fmaf(x0.x, x0.x, nrm);
@@ -1373,8 +1374,11 @@ int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* const* nets, int32_t tables, const f
     // the sorted order stays wherever the last pass left it (no copy back)
     uint32_t *skeys, *ids;
     HDB_CALL(radix_sort_u32_u32_out(ctx, keys, order, n, 0, 24, &skeys, &ids));
-    hipLaunchKernelGGL(k_mlp_permute, dim3(grid_for(n * d, 256)), dim3(256), 0, ctx->stream, X, ids, n, d, Xs);
-    HDB_LAUNCHED();
+    const bool mfma_search = d == 12 && n >= window + 32 && (reinterpret_cast<uintptr_t>(X) & 15) == 0;
+    if (!mfma_search) {  // the other searches read the points in bucket order
+      hipLaunchKernelGGL(k_mlp_permute, dim3(grid_for(n * d, 256)), dim3(256), 0, ctx->stream, X, ids, n, d, Xs);
+      HDB_LAUNCHED();
+    }
     const int slot0 = t * keep;
 #define HDB_SEED(D_, W_, K_, ST_)                                                                                   \
   hipLaunchKernelGGL((k_mlp_seed<D_, W_, K_, ST_>), grid, dim3(kSeedPts), ST_ ? lds : 0, ctx->stream, Xs, ids, n, d, \
@@ -1409,12 +1413,12 @@ int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* const* nets, int32_t tables, const f
   do {                                                                                                        \
     if (keep == 16)                                                                                           \
       hipLaunchKernelGGL((k_mlp_seed_mfma<W_, 16>), gridm, dim3(64 * kSmWaves), seed_mfma_lds(W_), ctx->stream, \
-                         Xs, ids, n, slot0, cand);                                                            \
+                         X, ids, n, slot0, cand);                                                             \
     else                                                                                                      \
       hipLaunchKernelGGL((k_mlp_seed_mfma<W_, 8>), gridm, dim3(64 * kSmWaves), seed_mfma_lds(W_), ctx->stream,  \
-                         Xs, ids, n, slot0, cand);                                                            \
+                         X, ids, n, slot0, cand);                                                             \
   } while (0)
-    if (d == 12 && n >= window + 32) {
+    if (mfma_search) {
       if (window == 128) HDB_SEEDM(128);
       else if (window == 256) HDB_SEEDM(256);
       else HDB_SEEDM(512);
