@@ -478,24 +478,6 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
   p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
 }
 
-// Sort keys of one bucket table: (12 sign bits of the embedding z << 12) |
-// 12-bit rank of x.u (u a fixed unit direction, x.u clamped to [-lim, lim]).
-__global__ void k_mlp_keys(const float* __restrict__ Z, const float* __restrict__ X, int d,
-                           const float* __restrict__ u, float lim, int64_t n, uint32_t* __restrict__ keys,
-                           uint32_t* __restrict__ ids) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  uint32_t code = 0;
-  for (int b = 0; b < kCode; ++b) code |= (Z[i * kOutP + b] > 0.0f ? 1u : 0u) << b;
-  float s = 0.0f;
-  for (int k = 0; k < d; ++k) s = __builtin_fmaf(X[i * d + k], u[k], s);
-  float t = (s / lim) * 0.5f + 0.5f;
-  t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
-  const uint32_t q = static_cast<uint32_t>(t * 4095.0f);  // 12 bits: a 24-bit key sorts in three passes
-  keys[i] = (code << 12) | q;
-  ids[i] = static_cast<uint32_t>(i);
-}
-
 // ------------------------------------------------------- inference path --
 // Weights packed into MFMA fragment order (BF16): fragment f holds, for lane
 // l, the 8 weights W[k0 + 8*(l/16) + e][n0 + l%16] (e = 0..7) of column tile
@@ -529,12 +511,34 @@ __global__ void k_mlp_pack(const float* __restrict__ prm, MlpLayout L, __bf16* _
   }
 }
 
+// Bucket keys written by the inference kernel in place of Z: key =
+// (sign code of outputs 0..11) << 12 | the projection x.u quantised to 12
+// bits over [-lim, lim].
+struct InferKeys {
+  const float* u;
+  float lim;
+  uint32_t* keys;
+  uint32_t* ids;
+};
+
+__device__ __forceinline__ uint32_t proj_q12(const float* __restrict__ x, const float* __restrict__ u, int d,
+                                             float lim) {
+  float s = 0.0f;
+  for (int k = 0; k < d; ++k) s = __builtin_fmaf(x[k], u[k], s);
+  float t = (s / lim) * 0.5f + 0.5f;
+  t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+  return static_cast<uint32_t>(t * 4095.0f);  // 12 bits: a 24-bit key sorts in three passes
+}
+
 // Z = forward(X) for R rows: the block keeps the packed weights and biases in
 // LDS and each wave strides over 16-row tiles, with activations stored as
-// BF16 (the value they take as the next layer's MFMA operand).
+// BF16 (the value they take as the next layer's MFMA operand).  KEYS: the
+// rows' bucket keys instead of Z (each row's 12 code bits are one ballot of
+// the lanes holding its outputs).
+template <bool KEYS>
 __global__ __launch_bounds__(1024) void k_mlp_infer(const float* __restrict__ X, int d, int64_t R,
                                                    const __bf16* __restrict__ packed, const float* __restrict__ prm,
-                                                   MlpLayout L, float* __restrict__ Z) {
+                                                   MlpLayout L, float* __restrict__ Z, InferKeys ko) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_b[];
   const int ks1 = L.kinp / 32;
   const int nfrag = infer_frags(L.kinp);
@@ -622,10 +626,27 @@ __global__ __launch_bounds__(1024) void k_mlp_infer(const float* __restrict__ X,
         const bf16x8 b = *reinterpret_cast<const bf16x8*>(sw + ((ks1 * 8 + 32 + ks) * 64 + l) * 8);
         acc = mfma(a, b, acc);
       }
+      if (KEYS) {
+        // lane 16 kq + c holds output c of rows 4 kq + q
+        uint64_t m[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int64_t row = r0 + 4 * kq + q;
-        if (row < R) Z[row * kOutP + r16] = acc[q] + sb[2 * kHid + r16];
+        for (int q = 0; q < 4; ++q) m[q] = __ballot(r16 < kCode && acc[q] + sb[2 * kHid + r16] > 0.0f);
+        const int64_t row = r0 + 4 * kq + r16;
+        if (r16 < 4 && row < R) {
+          uint64_t mm = m[0];
+#pragma unroll
+          for (int q = 1; q < 4; ++q)
+            if (r16 == q) mm = m[q];
+          const uint32_t code = static_cast<uint32_t>(mm >> (16 * kq)) & ((1u << kCode) - 1u);
+          ko.keys[row] = (code << 12) | proj_q12(X + row * d, ko.u, d, ko.lim);
+          ko.ids[row] = static_cast<uint32_t>(row);
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int64_t row = r0 + 4 * kq + q;
+          if (row < R) Z[row * kOutP + r16] = acc[q] + sb[2 * kHid + r16];
+        }
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1070,7 +1091,7 @@ int mlp_forward(hdb_ctx* ctx, hdb_mlp* net, const float* X, int d, const int32_t
 }
 
 // Inference forward Z = f(X) over R rows (packed-weight kernel).
-int mlp_infer(hdb_ctx* ctx, hdb_mlp* net, const float* X, int d, int64_t R, float* Z) {
+int mlp_infer(hdb_ctx* ctx, hdb_mlp* net, const float* X, int d, int64_t R, float* Z, const InferKeys* ko) {
   const MlpLayout& L = net->L;
   const int nfrag = infer_frags(L.kinp);
   __bf16* packed;
@@ -1083,7 +1104,12 @@ int mlp_infer(hdb_ctx* ctx, hdb_mlp* net, const float* X, int d, int64_t R, floa
                      static_cast<size_t>(waves) * (16 * (L.kinp + 8) + 16 * (kHid + 8)) * 2;
   const int64_t tiles = (R + 15) / 16;
   const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>((tiles + waves - 1) / waves, 512)));
-  hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(waves * 64), lds, ctx->stream, X, d, R, packed, net->prm, L, Z);
+  if (ko)
+    hipLaunchKernelGGL(k_mlp_infer<true>, dim3(grid), dim3(waves * 64), lds, ctx->stream, X, d, R, packed, net->prm, L,
+                       Z, *ko);
+  else
+    hipLaunchKernelGGL(k_mlp_infer<false>, dim3(grid), dim3(waves * 64), lds, ctx->stream, X, d, R, packed, net->prm,
+                       L, Z, InferKeys{nullptr, 1.0f, nullptr, nullptr});
   HDB_LAUNCHED();
   return HDB_OK;
 }
@@ -1257,7 +1283,7 @@ int hdb_mlp_forward(hdb_ctx* ctx, hdb_mlp* net, const float* X, int64_t m, int32
   if (!ctx || !net || !X || !Z || m < 0 || d != net->L.kin) return ctx ? ctx->fail(HDB_EINVAL, "bad forward args") : 1;
   HDB_HIP(hipSetDevice(ctx->device));
   if (m == 0) return HDB_OK;
-  return mlp_infer(ctx, net, X, d, m, Z);
+  return mlp_infer(ctx, net, X, d, m, Z, nullptr);
 }
 
 int hdb_mlp_loss_grad(hdb_ctx* ctx, hdb_mlp* net, const float* X, int64_t n, int32_t d, const int32_t* rows,
@@ -1366,11 +1392,9 @@ int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* const* nets, int32_t tables, const f
   const dim3 grid(static_cast<unsigned>(grid_for(n, kSeedPts)));
   const dim3 gridm(static_cast<unsigned>(grid_for(n, kSmPts)));
   for (int t = 0; t < tables; ++t) {
-    HDB_CALL(mlp_infer(ctx, nets[t], X, d, n, Z));
     // x.u range: 64 covers the spread of a unit direction over typical data
-    hipLaunchKernelGGL(k_mlp_keys, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, Z, X, d, dirs + t * d, 64.0f,
-                       n, keys, order);
-    HDB_LAUNCHED();
+    const InferKeys ko{dirs + t * d, 64.0f, keys, order};
+    HDB_CALL(mlp_infer(ctx, nets[t], X, d, n, Z, &ko));
     // the sorted order stays wherever the last pass left it (no copy back)
     uint32_t *skeys, *ids;
     HDB_CALL(radix_sort_u32_u32_out(ctx, keys, order, n, 0, 24, &skeys, &ids));
