@@ -1,5 +1,6 @@
 // extern "C" entry points of libhdbscan_mi355x.so: context, scratch arena,
 // stage timer and the hot-path functions declared in include/hdbscan_mi355x.h.
+#include <utility>
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -14,6 +15,11 @@ int hip_fail(hdb_ctx* ctx, hipError_t e, const char* what, const char* file, int
   std::snprintf(buf, sizeof(buf), "%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
   ctx->err = buf;
   return e == hipErrorOutOfMemory ? HDB_ENOMEM : HDB_EDEVICE;
+}
+
+void scratch_swap(hdb_ctx* ctx, Slot a, Slot b) {
+  std::swap(ctx->slot_ptr[a], ctx->slot_ptr[b]);
+  std::swap(ctx->slot_bytes[a], ctx->slot_bytes[b]);
 }
 
 int scratch(hdb_ctx* ctx, Slot slot, size_t bytes, void** out) {

@@ -386,8 +386,12 @@ int mrd_prep(hdb_ctx* ctx, const hdb_nnd_element* lists, int64_t n, int k, int32
   int32_t* re = cnt + 4 * n;
   hipLaunchKernelGGL(k_rev_emit, dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream, lists, n, k, key, rev);
   HDB_LAUNCHED();
-  HDB_CALL(radix_sort_u32_u32(ctx, key, rev, m, 0, bits_for(n)));
-  HDB_CALL(segment_bounds(ctx, key, m, n, rs, re));
+  // an odd number of passes leaves the sorted pairs in the sort's scratch:
+  // that buffer then becomes the row slot mrd_rows reads (no copy back)
+  uint32_t *skey, *srev;
+  HDB_CALL(radix_sort_u32_u32_out(ctx, key, rev, m, 0, bits_for(n), &skey, &srev));
+  if (srev != rev) scratch_swap(ctx, kSlotRowTmp, kSlotSortVals);
+  HDB_CALL(segment_bounds(ctx, skey, m, n, rs, re));
   if (counts) {
     hipLaunchKernelGGL(k_seg_len, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, rs, re, n, counts);
     HDB_LAUNCHED();

@@ -95,6 +95,9 @@ int hip_fail(hdb_ctx* ctx, hipError_t e, const char* what, const char* file, int
 
 // Scratch buffer of at least `bytes` for `slot`; contents are undefined.
 int scratch(hdb_ctx* ctx, Slot slot, size_t bytes, void** out);
+// Exchange two slots' buffers (e.g. to keep a sort's output where its
+// input lived, without a copy).
+void scratch_swap(hdb_ctx* ctx, Slot a, Slot b);
 template <typename T>
 inline int scratch_t(hdb_ctx* ctx, Slot slot, size_t count, T** out) {
   return scratch(ctx, slot, count * sizeof(T), reinterpret_cast<void**>(out));
