@@ -512,22 +512,35 @@ __global__ void k_mlp_pack(const float* __restrict__ prm, MlpLayout L, __bf16* _
 }
 
 // Bucket keys written by the inference kernel in place of Z: key =
-// (sign code of outputs 0..11) << 12 | the projection x.u quantised to 12
-// bits over [-lim, lim].
+// (sign code of outputs 0..11) << qbits | the projection x.u quantised to
+// qbits bits over [-lim, lim].
 struct InferKeys {
   const float* u;
   float lim;
+  int qbits;
   uint32_t* keys;
   uint32_t* ids;
 };
 
-__device__ __forceinline__ uint32_t proj_q12(const float* __restrict__ x, const float* __restrict__ u, int d,
-                                             float lim) {
+__device__ __forceinline__ uint32_t proj_q(const float* __restrict__ x, const float* __restrict__ u, int d,
+                                           float lim, int qbits) {
   float s = 0.0f;
   for (int k = 0; k < d; ++k) s = __builtin_fmaf(x[k], u[k], s);
   float t = (s / lim) * 0.5f + 0.5f;
   t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
-  return static_cast<uint32_t>(t * 4095.0f);  // 12 bits: a 24-bit key sorts in three passes
+  return static_cast<uint32_t>(t * static_cast<float>((1u << qbits) - 1u));
+}
+
+// Projection bits of the bucket keys: enough levels that a level holds about
+// 2^8 points or fewer at n points, 12 at least and 20 at most, so a key of
+// 12 + qbits bits sorts in three passes up to about a million points and in
+// four beyond.  The order inside a projection level is arbitrary, so coarse
+// levels at large n fill a window with unrelated points.
+inline int proj_bits(int64_t n) {
+  int lg = 0;
+  while ((int64_t{1} << lg) < n) ++lg;
+  const int q = lg - 8;
+  return q < 12 ? 12 : (q > 20 ? 20 : q);
 }
 
 // Z = forward(X) for R rows: the block keeps the packed weights and biases in
@@ -638,7 +651,7 @@ __global__ __launch_bounds__(1024) void k_mlp_infer(const float* __restrict__ X,
           for (int q = 1; q < 4; ++q)
             if (r16 == q) mm = m[q];
           const uint32_t code = static_cast<uint32_t>(mm >> (16 * kq)) & ((1u << kCode) - 1u);
-          ko.keys[row] = (code << 12) | proj_q12(X + row * d, ko.u, d, ko.lim);
+          ko.keys[row] = (code << ko.qbits) | proj_q(X + row * d, ko.u, d, ko.lim, ko.qbits);
           ko.ids[row] = static_cast<uint32_t>(row);
         }
       } else {
@@ -1109,7 +1122,7 @@ int mlp_infer(hdb_ctx* ctx, hdb_mlp* net, const float* X, int d, int64_t R, floa
                        Z, *ko);
   else
     hipLaunchKernelGGL(k_mlp_infer<false>, dim3(grid), dim3(waves * 64), lds, ctx->stream, X, d, R, packed, net->prm,
-                       L, Z, InferKeys{nullptr, 1.0f, nullptr, nullptr});
+                       L, Z, InferKeys{nullptr, 1.0f, 12, nullptr, nullptr});
   HDB_LAUNCHED();
   return HDB_OK;
 }
@@ -1360,6 +1373,7 @@ int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* const* nets, int32_t tables, const f
   HDB_HIP(hipSetDevice(ctx->device));
   Stage st(ctx, "learned_init");
   const int keep = tables <= 4 ? 16 : 8;  // nearest window entries kept per table
+  const int qbits = proj_bits(n);
   float *Z, *dirs, *Xs;
   uint32_t *keys, *order;
   uint64_t* cand;
@@ -1393,11 +1407,11 @@ int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* const* nets, int32_t tables, const f
   const dim3 gridm(static_cast<unsigned>(grid_for(n, kSmPts)));
   for (int t = 0; t < tables; ++t) {
     // x.u range: 64 covers the spread of a unit direction over typical data
-    const InferKeys ko{dirs + t * d, 64.0f, keys, order};
+    const InferKeys ko{dirs + t * d, 64.0f, qbits, keys, order};
     HDB_CALL(mlp_infer(ctx, nets[t], X, d, n, Z, &ko));
     // the sorted order stays wherever the last pass left it (no copy back)
     uint32_t *skeys, *ids;
-    HDB_CALL(radix_sort_u32_u32_out(ctx, keys, order, n, 0, 24, &skeys, &ids));
+    HDB_CALL(radix_sort_u32_u32_out(ctx, keys, order, n, 0, kCode + qbits, &skeys, &ids));
     const bool mfma_search = d == 12 && n >= window + 32 && (reinterpret_cast<uintptr_t>(X) & 15) == 0;
     if (!mfma_search) {  // the other searches read the points in bucket order
       hipLaunchKernelGGL(k_mlp_permute, dim3(grid_for(n * d, 256)), dim3(256), 0, ctx->stream, X, ids, n, d, Xs);
