@@ -221,10 +221,6 @@ __global__ void k_seg_len(const int32_t* __restrict__ rs, const int32_t* __restr
   if (v < n) counts[v] = re[v] - rs[v];
 }
 
-__device__ __forceinline__ int64_t clique_share(int64_t v, const int32_t* ahrank, int64_t h) {
-  return ahrank[v] < 0 ? 0 : h - 1;
-}
-
 // One wave per row: own ids sorted and made unique (own_sorted, uown), the
 // reverse segment compacted in place to the unique ids that are not own ids
 // (urev), and the row length with the clique share (total).
@@ -241,9 +237,15 @@ __global__ __launch_bounds__(256) void k_row_count(const hdb_nnd_element* __rest
   if (v >= n) return;  // uniform across the wave; no block barriers below
   const uint64_t lt = lanemask_lt();
   constexpr uint32_t kNoId = 0xFFFFFFFFu;  // ids are below 2^31
+  // the list entry, the reverse segment's bounds and then its first 64 ids
+  // are loaded ahead of the sort, so their latency overlaps it
   uint32_t x = kNoId;
+  const int32_t lab = static_cast<int>(lane) < k ? lists[v * k + lane].label : 0;
+  const int64_t b0 = rs[v], b1 = re[v];
+  const int32_t ahr = ahrank[v];
+  const uint32_t y0 = b0 + static_cast<int64_t>(lane) < b1 ? rev[b0 + lane] : 0xFFFFFFFFu;
   if (static_cast<int>(lane) < k) {
-    const uint32_t id = label_id(lists[v * k + lane].label);
+    const uint32_t id = label_id(lab);
     if (id != kInvalidId && static_cast<int64_t>(id) < n && static_cast<int64_t>(id) != v) x = id;
   }
   x = bitonic_group_u32<64>(x);
@@ -258,12 +260,11 @@ __global__ __launch_bounds__(256) void k_row_count(const hdb_nnd_element* __rest
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-  const int64_t b0 = rs[v], b1 = re[v];
   int64_t ur = 0;
   uint32_t carry = 0xFFFFFFFFu;
   for (int64_t c = b0; c < b1; c += 64) {
     const int64_t p = c + lane;
-    const uint32_t y = p < b1 ? rev[p] : 0xFFFFFFFFu;
+    const uint32_t y = c == b0 ? y0 : (p < b1 ? rev[p] : 0xFFFFFFFFu);
     uint32_t py = __shfl(y, static_cast<int>((lane + 63) & 63));
     if (lane == 0) py = carry;
     bool keep = p < b1 && y != py && static_cast<int64_t>(y) != v;
@@ -283,7 +284,7 @@ __global__ __launch_bounds__(256) void k_row_count(const hdb_nnd_element* __rest
   if (lane == 0) {
     uown[v] = uo;
     urev[v] = static_cast<int32_t>(ur);
-    total[v] = static_cast<int32_t>(uo + ur + clique_share(v, ahrank, h));
+    total[v] = static_cast<int32_t>(uo + ur + (ahr < 0 ? 0 : h - 1));
   }
 }
 
@@ -317,12 +318,19 @@ __global__ __launch_bounds__(256) void k_row_fill(const float* __restrict__ X, i
   const unsigned wv = threadIdx.x >> 6;
   const int64_t v = static_cast<int64_t>(blockIdx.x) * 4 + wv;
   if (v >= n) return;  // uniform across the wave; no block barriers below
+  // the vertex's scalars are loaded together, ahead of the row staging
   const int64_t uo = uown[v];
   const int64_t ur = urev[v];
+  const int32_t rsv = rs[v];
+  const int64_t out0 = nindex[v];
+  const float cv = core[v];
+  const int32_t r = ahrank[v];
   // the row's own and reverse ids (both sorted) go to LDS, so the rank
-  // searches below read LDS; a longer reverse run is searched in place
-  const uint32_t* rvg = rev + rs[v];
-  if (static_cast<int64_t>(lane) < uo) s_own[wv][lane] = static_cast<uint32_t>(own_sorted[v * k + lane]);
+  // searches below read LDS; a longer reverse run is searched in place.
+  // Own ids number at most k <= 64: lane a keeps own id a in a register.
+  const uint32_t* rvg = rev + rsv;
+  const uint32_t xo = static_cast<int64_t>(lane) < uo ? static_cast<uint32_t>(own_sorted[v * k + lane]) : 0u;
+  if (static_cast<int64_t>(lane) < uo) s_own[wv][lane] = xo;
   if (ur <= kRevLds)
     for (int64_t b = lane; b < ur; b += 64) s_rev[wv][b] = rvg[b];
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -330,22 +338,32 @@ __global__ __launch_bounds__(256) void k_row_fill(const float* __restrict__ X, i
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   const uint32_t* own = s_own[wv];
   const uint32_t* rv = ur <= kRevLds ? s_rev[wv] : rvg;
-  const int64_t out0 = nindex[v];
-  const float cv = core[v];
-  const int32_t r = ahrank[v];
   // clique ids < x, without v itself
   auto clique_before = [&](uint32_t x) -> int64_t {
     if (r < 0) return 0;
     const int64_t lo = rank_in<true>(ah, h, x);
     return lo - (r < lo ? 1 : 0);
   };
-  for (int64_t a = lane; a < uo; a += 64) {
-    const uint32_t x = static_cast<uint32_t>(own[a]);
-    const int64_t pos = out0 + a + rank_in<true>(rv, ur, x) + clique_before(x);
-    nlist[pos] = static_cast<int32_t>(x);
-    w[pos] = fmaxf(cv, core[x]);
+  // own entry `lane` and reverse entries lane, lane + 64, ...: the first
+  // reverse entry's core gather is issued with the own entry's
+  {
+    const bool has_o = static_cast<int64_t>(lane) < uo;
+    const bool has_r = static_cast<int64_t>(lane) < ur;
+    const uint32_t y = has_r ? rv[lane] : 0u;
+    const float co = has_o ? core[xo] : 0.0f;
+    const float cr = has_r ? core[y] : 0.0f;
+    if (has_o) {
+      const int64_t pos = out0 + lane + rank_in<true>(rv, ur, xo) + clique_before(xo);
+      nlist[pos] = static_cast<int32_t>(xo);
+      w[pos] = fmaxf(cv, co);
+    }
+    if (has_r) {
+      const int64_t pos = out0 + lane + rank_in<true>(own, uo, y) + clique_before(y);
+      nlist[pos] = static_cast<int32_t>(y);
+      w[pos] = fmaxf(cv, cr);
+    }
   }
-  for (int64_t b = lane; b < ur; b += 64) {
+  for (int64_t b = lane + 64; b < ur; b += 64) {
     const uint32_t y = rv[b];
     const int64_t pos = out0 + b + rank_in<true>(own, uo, y) + clique_before(y);
     nlist[pos] = static_cast<int32_t>(y);
