@@ -989,10 +989,13 @@ __global__ __launch_bounds__(256) void k_nnd_update(hdb_nnd_element* __restrict_
           // cannot change the list: drop it, and skip the merge when none is
           // left.
           const uint64_t pk = pr >> 1;
-          int pos = 0;  // entries below the proposal, capped at 31
-#pragma unroll
-          for (int step = 16; step > 0; step >>= 1)
-            if ((lst[pos + step - 1] >> 1) < pk) pos += step;
+          // entries below the proposal, capped at 31, in three dependent
+          // rounds of LDS reads (an 8-entry block, a pair within it, then
+          // one entry) where a binary search takes five
+          int pos = ((lst[7] >> 1) < pk ? 8 : 0) + ((lst[15] >> 1) < pk ? 8 : 0) + ((lst[23] >> 1) < pk ? 8 : 0);
+          pos += ((lst[pos + 1] >> 1) < pk ? 2 : 0) + ((lst[pos + 3] >> 1) < pk ? 2 : 0) +
+                 ((lst[pos + 5] >> 1) < pk ? 2 : 0);
+          if ((lst[pos] >> 1) < pk) pos += 1;
           // lst[pos] >= pk unless all 32 entries are below it (pos = 31)
           const bool keep = pr != kKeyMax && (lst[pos] >> 1) > pk;
           const uint64_t km = __ballot(keep);
