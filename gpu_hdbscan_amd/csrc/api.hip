@@ -108,10 +108,14 @@ int hdb_ctx_create(hdb_ctx** out, int device) {
   void* pin = nullptr;
   if (hipHostMalloc(&pin, kPinnedWords * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
       hipEventCreateWithFlags(&ctx->ready[0], hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&ctx->ready[1], hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&ctx->ready[1], hipEventDisableTiming) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming) != hipSuccess) {
     if (pin) (void)hipHostFree(pin);
-    if (ctx->ready[0]) (void)hipEventDestroy(ctx->ready[0]);
-    if (ctx->ready[1]) (void)hipEventDestroy(ctx->ready[1]);
+    for (hipEvent_t e : {ctx->ready[0], ctx->ready[1], ctx->fork_ev, ctx->join_ev})
+      if (e) (void)hipEventDestroy(e);
+    if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return HDB_EDEVICE;
@@ -134,6 +138,12 @@ int hdb_ctx_destroy(hdb_ctx* ctx) {
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   for (hipEvent_t e : ctx->ready)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : {ctx->fork_ev, ctx->join_ev})
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->stream2) {
+    (void)hipStreamSynchronize(ctx->stream2);
+    (void)hipStreamDestroy(ctx->stream2);
+  }
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return HDB_OK;

@@ -33,6 +33,9 @@ enum Slot : int {
   kSlotPipeNindex, kSlotPipeNlist, kSlotPipeW, kSlotPipeAh,  // pipeline
   kSlotBfTmp, kSlotMisc0, kSlotMisc1, kSlotMisc2,
   kSlotPropTgt, kSlotPropSE, kSlotRevSE, kSlotPart,                     // sort-routed NN-Descent
+  // second set of the per-table learned-init slots (keys, sort, packed
+  // weights), used by the work on the context's second stream
+  kSlotAlt0, kSlotAlt1, kSlotAlt2, kSlotAlt3, kSlotAlt4, kSlotAlt5, kSlotAlt6, kSlotAlt7, kSlotAlt8, kSlotAlt9,
   kSlotCount
 };
 
@@ -62,6 +65,10 @@ struct hdb_ctx {
   // Pinned host words and events for readbacks that overlap later work.
   int64_t* pinned = nullptr;  // kPinnedWords, from hipHostMalloc
   hipEvent_t ready[2] = {nullptr, nullptr};
+  // A second stream (default priority) for independent work inside one call;
+  // every call joins it back into `stream` before returning.
+  hipStream_t stream2 = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   // Pipeline CSR (kept when requested, for tests).
   int64_t csr_edges = 0;
 

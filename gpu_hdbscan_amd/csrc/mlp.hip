@@ -1077,6 +1077,58 @@ struct hdb_mlp {
 
 namespace {
 
+// Work on the context's second stream inside one call: begin() makes
+// stream2 wait for everything queued on `stream`; join() (also run at scope
+// exit, so an early error return leaves no work outstanding on stream2)
+// makes `stream` wait for stream2.
+struct StreamFork {
+  hdb_ctx* ctx;
+  bool forked = false;
+  explicit StreamFork(hdb_ctx* c) : ctx(c) {}
+  int begin() {
+    HDB_HIP(hipEventRecord(ctx->fork_ev, ctx->stream));
+    HDB_HIP(hipStreamWaitEvent(ctx->stream2, ctx->fork_ev, 0));
+    forked = true;
+    return HDB_OK;
+  }
+  int join() {
+    if (!forked) return HDB_OK;
+    forked = false;
+    HDB_HIP(hipEventRecord(ctx->join_ev, ctx->stream2));
+    HDB_HIP(hipStreamWaitEvent(ctx->stream, ctx->join_ev, 0));
+    return HDB_OK;
+  }
+  ~StreamFork() { (void)join(); }
+};
+
+// While alive (and enabled): launches go to stream2 and the per-table slots
+// (keys, order, sort and scan scratch, packed weights, permuted points) are
+// the second set, so they never alias the buffers of work on `stream`.
+struct AltSlots {
+  hdb_ctx* ctx;
+  hipStream_t saved;
+  bool on;
+  static void swap_all(hdb_ctx* c) {
+    static const Slot a[] = {kSlotMisc0, kSlotMisc1, kSlotMisc2, kSlotSortKeys, kSlotSortVals,
+                             kSlotSortHist, kSlotSortAux, kSlotScanTmp, kSlotMlpPack, kSlotSeedX};
+    static const Slot b[] = {kSlotAlt0, kSlotAlt1, kSlotAlt2, kSlotAlt3, kSlotAlt4,
+                             kSlotAlt5, kSlotAlt6, kSlotAlt7, kSlotAlt8, kSlotAlt9};
+    for (int i = 0; i < 10; ++i) scratch_swap(c, a[i], b[i]);
+  }
+  AltSlots(hdb_ctx* c, bool enable) : ctx(c), saved(c->stream), on(enable) {
+    if (on) {
+      ctx->stream = ctx->stream2;
+      swap_all(ctx);
+    }
+  }
+  ~AltSlots() {
+    if (on) {
+      swap_all(ctx);
+      ctx->stream = saved;
+    }
+  }
+};
+
 int mlp_alloc(hdb_ctx* ctx, void** p, size_t bytes) {
   if (hipMalloc(p, bytes) != hipSuccess) {
     (void)hipGetLastError();
@@ -1374,14 +1426,9 @@ int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* const* nets, int32_t tables, const f
   Stage st(ctx, "learned_init");
   const int keep = tables <= 4 ? 16 : 8;  // nearest window entries kept per table
   const int qbits = proj_bits(n);
-  float *Z, *dirs, *Xs;
-  uint32_t *keys, *order;
+  float* dirs;
   uint64_t* cand;
-  HDB_CALL(scratch_t(ctx, kSlotMisc0, n * kOutP, &Z));
-  HDB_CALL(scratch_t(ctx, kSlotMisc1, n, &keys));
-  HDB_CALL(scratch_t(ctx, kSlotMisc2, n, &order));
   HDB_CALL(scratch_t(ctx, kSlotBfTmp, static_cast<int64_t>(tables) * d, &dirs));
-  HDB_CALL(scratch_t(ctx, kSlotSeedX, n * d, &Xs));
   HDB_CALL(scratch_t(ctx, kSlotSeedCand, n * kSeedSlots, &cand));
   // per table: a unit direction for the order inside a bucket, Box-Muller
   // from xorshift64*
@@ -1405,15 +1452,27 @@ int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* const* nets, int32_t tables, const f
   const bool staged = lds <= 80 * 1024;
   const dim3 grid(static_cast<unsigned>(grid_for(n, kSeedPts)));
   const dim3 gridm(static_cast<unsigned>(grid_for(n, kSmPts)));
+  const bool mfma_search = d == 12 && n >= window + 32 && (reinterpret_cast<uintptr_t>(X) & 15) == 0;
+  // Tables alternate between the context's two streams, each with its own
+  // keys, sort scratch and packed weights, so one table's sort (a few
+  // hundred blocks) overlaps the other's network and window search; the
+  // window searches write disjoint candidate slots.
+  StreamFork fork(ctx);
+  HDB_CALL(fork.begin());
   for (int t = 0; t < tables; ++t) {
+    AltSlots alt(ctx, (t & 1) != 0);
+    uint32_t *keys, *order;
+    float* Xs = nullptr;
+    HDB_CALL(scratch_t(ctx, kSlotMisc1, n, &keys));
+    HDB_CALL(scratch_t(ctx, kSlotMisc2, n, &order));
     // x.u range: 64 covers the spread of a unit direction over typical data
     const InferKeys ko{dirs + t * d, 64.0f, qbits, keys, order};
-    HDB_CALL(mlp_infer(ctx, nets[t], X, d, n, Z, &ko));
+    HDB_CALL(mlp_infer(ctx, nets[t], X, d, n, nullptr, &ko));
     // the sorted order stays wherever the last pass left it (no copy back)
     uint32_t *skeys, *ids;
     HDB_CALL(radix_sort_u32_u32_out(ctx, keys, order, n, 0, kCode + qbits, &skeys, &ids));
-    const bool mfma_search = d == 12 && n >= window + 32 && (reinterpret_cast<uintptr_t>(X) & 15) == 0;
     if (!mfma_search) {  // the other searches read the points in bucket order
+      HDB_CALL(scratch_t(ctx, kSlotSeedX, n * d, &Xs));
       hipLaunchKernelGGL(k_mlp_permute, dim3(grid_for(n * d, 256)), dim3(256), 0, ctx->stream, X, ids, n, d, Xs);
       HDB_LAUNCHED();
     }
@@ -1472,6 +1531,7 @@ int hdb_learned_init(hdb_ctx* ctx, hdb_mlp* const* nets, int32_t tables, const f
 #undef HDB_SEED
     HDB_LAUNCHED();
   }
+  HDB_CALL(fork.join());
   hipLaunchKernelGGL(k_mlp_merge, dim3(grid_for(n, 4)), dim3(256), 0, ctx->stream, cand, X, n, d, seed, lists);
   HDB_LAUNCHED();
   return HDB_OK;
