@@ -35,7 +35,7 @@ enum Slot : int {
   kSlotPropTgt, kSlotPropSE, kSlotRevSE, kSlotPart,                     // sort-routed NN-Descent
   // second set of the per-table learned-init slots (keys, sort, packed
   // weights), used by the work on the context's second stream
-  kSlotAlt0, kSlotAlt1, kSlotAlt2, kSlotAlt3, kSlotAlt4, kSlotAlt5, kSlotAlt6, kSlotAlt7, kSlotAlt8, kSlotAlt9,
+  kSlotAlt0, kSlotAlt1, kSlotAlt2, kSlotAlt3, kSlotAlt4, kSlotAlt5, kSlotAlt6, kSlotAlt7, kSlotAlt8,
   kSlotCount
 };
 

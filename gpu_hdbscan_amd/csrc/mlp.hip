@@ -1109,11 +1109,12 @@ struct AltSlots {
   hipStream_t saved;
   bool on;
   static void swap_all(hdb_ctx* c) {
-    static const Slot a[] = {kSlotMisc0, kSlotMisc1, kSlotMisc2, kSlotSortKeys, kSlotSortVals,
-                             kSlotSortHist, kSlotSortAux, kSlotScanTmp, kSlotMlpPack, kSlotSeedX};
+    static const Slot a[] = {kSlotMisc1,    kSlotMisc2,   kSlotSortKeys, kSlotSortVals, kSlotSortHist,
+                             kSlotSortAux, kSlotScanTmp, kSlotMlpPack,  kSlotSeedX};
     static const Slot b[] = {kSlotAlt0, kSlotAlt1, kSlotAlt2, kSlotAlt3, kSlotAlt4,
-                             kSlotAlt5, kSlotAlt6, kSlotAlt7, kSlotAlt8, kSlotAlt9};
-    for (int i = 0; i < 10; ++i) scratch_swap(c, a[i], b[i]);
+                             kSlotAlt5, kSlotAlt6, kSlotAlt7, kSlotAlt8};
+    static_assert(sizeof(a) == sizeof(b), "slot pairs");
+    for (size_t i = 0; i < sizeof(a) / sizeof(a[0]); ++i) scratch_swap(c, a[i], b[i]);
   }
   AltSlots(hdb_ctx* c, bool enable) : ctx(c), saved(c->stream), on(enable) {
     if (on) {

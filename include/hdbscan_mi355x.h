@@ -13,7 +13,9 @@
  *   - Every entry point returns an hdb_status; it never exits the process.
  *     hdb_last_error(ctx) returns a message for the last failure on ctx.
  *   - Work is issued on the context's stream; entry points that return
- *     sizes to the host synchronise that stream.
+ *     sizes to the host synchronise that stream.  A context also owns a
+ *     second stream for independent work inside one call, joined back into
+ *     the first before the call returns.  Both use the default priority.
  *   - One context is not thread-safe; use one context per host thread.
  *
  * Reference layouts kept bit-identical (tools/nndescent_element.cuh:9-57,
