@@ -261,6 +261,32 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
+// The value of lane (lane ^ j).  For j < 32 a ds_swizzle in bit mode
+// (and 0x1f, xor j) -- no address operand to form; for j = 32 a bpermute.
+__device__ __forceinline__ uint32_t lane_xor_u32(uint32_t v, int j) {
+  switch (j) {
+    case 1: return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(static_cast<int>(v), 0x1f | (1 << 10)));
+    case 2: return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(static_cast<int>(v), 0x1f | (2 << 10)));
+    case 4: return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(static_cast<int>(v), 0x1f | (4 << 10)));
+    case 8: return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(static_cast<int>(v), 0x1f | (8 << 10)));
+    case 16: return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(static_cast<int>(v), 0x1f | (16 << 10)));
+    default: return static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), j));
+  }
+}
+__device__ __forceinline__ uint64_t lane_xor_u64(uint64_t v, int j) {
+  const uint32_t lo = lane_xor_u32(static_cast<uint32_t>(v), j);
+  const uint32_t hi = lane_xor_u32(static_cast<uint32_t>(v >> 32), j);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// One compare-exchange of a bitonic network: the lane keeps the smaller of
+// (v, o) when want_min, else the larger, i.e. it takes o exactly when
+// (o < v) == want_min -- one compare and one select.
+template <typename T>
+__device__ __forceinline__ T bitonic_ce(T v, T o, bool want_min) {
+  return ((o < v) == want_min) ? o : v;
+}
+
 // Bitonic sort (ascending) of one u64 per lane within aligned groups of
 // GROUP lanes (GROUP = 32 or 64).
 template <int GROUP>
@@ -270,11 +296,9 @@ __device__ __forceinline__ uint64_t bitonic_group(uint64_t v) {
   for (int k = 2; k <= GROUP; k <<= 1) {
 #pragma unroll
     for (int j = k >> 1; j > 0; j >>= 1) {
-      uint64_t o = shfl_xor_u64(v, j);
-      bool up = (k == GROUP) ? true : ((lane & k) == 0);
-      bool low = (lane & j) == 0;
-      uint64_t mn = v < o ? v : o, mx = v < o ? o : v;
-      v = (low == up) ? mn : mx;
+      const bool up = (k == GROUP) ? true : ((lane & k) == 0);
+      const bool low = (lane & j) == 0;
+      v = bitonic_ce(v, lane_xor_u64(v, j), low == up);
     }
   }
   return v;
@@ -286,11 +310,9 @@ __device__ __forceinline__ uint32_t bitonic_group_u32(uint32_t v) {
   for (int k = 2; k <= GROUP; k <<= 1) {
 #pragma unroll
     for (int j = k >> 1; j > 0; j >>= 1) {
-      uint32_t o = __shfl_xor(v, j);
-      bool up = (k == GROUP) ? true : ((lane & k) == 0);
-      bool low = (lane & j) == 0;
-      uint32_t mn = v < o ? v : o, mx = v < o ? o : v;
-      v = (low == up) ? mn : mx;
+      const bool up = (k == GROUP) ? true : ((lane & k) == 0);
+      const bool low = (lane & j) == 0;
+      v = bitonic_ce(v, lane_xor_u32(v, j), low == up);
     }
   }
   return v;
@@ -307,19 +329,17 @@ __device__ __forceinline__ void bitonic128(uint64_t& r0, uint64_t& r1) {
       if (j == 64) {
         // partner is the other register of the same lane; i = lane (< 64)
         // and i ^ 64 = 64 + lane; k == 128 here, so ascending.
-        uint64_t mn = r0 < r1 ? r0 : r1, mx = r0 < r1 ? r1 : r0;
+        const uint64_t mn = r0 < r1 ? r0 : r1, mx = r0 < r1 ? r1 : r0;
         r0 = mn;
         r1 = mx;
       } else {
-        uint64_t o0 = shfl_xor_u64(r0, j), o1 = shfl_xor_u64(r1, j);
-        bool low = (lane & j) == 0;
+        const uint64_t o0 = lane_xor_u64(r0, j), o1 = lane_xor_u64(r1, j);
+        const bool low = (lane & j) == 0;
         // element indices lane (r0) and 64 + lane (r1)
-        bool up0 = (k == 128) ? true : ((lane & k) == 0);
-        bool up1 = (k == 128) ? true : (((64u + lane) & k) == 0);
-        uint64_t mn0 = r0 < o0 ? r0 : o0, mx0 = r0 < o0 ? o0 : r0;
-        uint64_t mn1 = r1 < o1 ? r1 : o1, mx1 = r1 < o1 ? o1 : r1;
-        r0 = (low == up0) ? mn0 : mx0;
-        r1 = (low == up1) ? mn1 : mx1;
+        const bool up0 = (k == 128) ? true : ((lane & k) == 0);
+        const bool up1 = (k == 128) ? true : (((64u + lane) & k) == 0);
+        r0 = bitonic_ce(r0, o0, low == up0);
+        r1 = bitonic_ce(r1, o1, low == up1);
       }
     }
   }
@@ -337,14 +357,12 @@ __device__ __forceinline__ void bitonic128_u32(uint32_t& r0, uint32_t& r1) {
         r0 = mn;
         r1 = mx;
       } else {
-        const uint32_t o0 = __shfl_xor(r0, j), o1 = __shfl_xor(r1, j);
+        const uint32_t o0 = lane_xor_u32(r0, j), o1 = lane_xor_u32(r1, j);
         const bool low = (lane & j) == 0;
         const bool up0 = (k == 128) ? true : ((lane & k) == 0);
         const bool up1 = (k == 128) ? true : (((64u + lane) & k) == 0);
-        const uint32_t mn0 = r0 < o0 ? r0 : o0, mx0 = r0 < o0 ? o0 : r0;
-        const uint32_t mn1 = r1 < o1 ? r1 : o1, mx1 = r1 < o1 ? o1 : r1;
-        r0 = (low == up0) ? mn0 : mx0;
-        r1 = (low == up1) ? mn1 : mx1;
+        r0 = bitonic_ce(r0, o0, low == up0);
+        r1 = bitonic_ce(r1, o1, low == up1);
       }
     }
   }

@@ -830,11 +830,7 @@ __global__ __launch_bounds__(64) void k_nnd_join_mfma(const float* __restrict__ 
 __device__ __forceinline__ uint64_t bitonic_merge64(uint64_t v) {
   const unsigned lane = __lane_id();
 #pragma unroll
-  for (int j = 32; j > 0; j >>= 1) {
-    const uint64_t o = shfl_xor_u64(v, j);
-    const uint64_t mn = v < o ? v : o, mx = v < o ? o : v;
-    v = (lane & j) == 0 ? mn : mx;
-  }
+  for (int j = 32; j > 0; j >>= 1) v = bitonic_ce(v, lane_xor_u64(v, j), (lane & j) == 0);
   return v;
 }
 
