@@ -911,12 +911,16 @@ __device__ __forceinline__ uint64_t insert_few(uint64_t cur, uint64_t pr, bool l
 constexpr int kGroupBits = 4;     // lists per update block: 16
 constexpr int kGroupTile = 1024;  // proposals staged per LDS tile
 
+constexpr int kUpdFull = 1;   // merge (and so deduplicate) every list, with or without proposals
+constexpr int kUpdStrip = 2;  // write labels without the old flag (the last iteration)
+
 // One block of 4 waves per group g of lists [16g, 16g + 16); prop_s/prop_e[g]
-// bound the group's proposals; each wave merges four lists.  full: merge
+// bound the group's proposals; each wave merges four lists.  kUpdFull: merge
 // (and so deduplicate) every list even without proposals (first iteration);
 // otherwise a list without proposals is only marked old.
 __global__ __launch_bounds__(256) void k_nnd_update(hdb_nnd_element* __restrict__ lists, int64_t n, NndBufs b,
-                                                    int full) {
+                                                    int flags) {
+  const bool full = (flags & kUpdFull) != 0;
   constexpr int G = 1 << kGroupBits;
   __shared__ uint64_t s_cur[G][32];
   __shared__ uint64_t s_key[kGroupTile];
@@ -1016,7 +1020,7 @@ __global__ __launch_bounds__(256) void k_nnd_update(hdb_nnd_element* __restrict_
     uint64_t cur = s_cur[q][lane & 31u];
     cur = lane < 32 ? cur : kKeyMax;
     if (full && !s_hit[q]) cur = merge_proposals(cur, kKeyMax, s_prop[w], s_top[w]);
-    if (lane < 32) lists[t * kK + lane] = key_to_elem(cur);
+    if (lane < 32) lists[t * kK + lane] = key_to_elem((flags & kUpdStrip) ? (cur | 1u) : cur);
     if (lane == 31) b.worst[t] = (cur == kKeyMax) ? __uint_as_float(0x7F800000u) : key_d2(cur);
   }
 }
@@ -1269,8 +1273,11 @@ static int iterate(hdb_ctx* ctx, const float* X, int64_t n, int d, int S, bool m
     }
     {
       Stage st(ctx, merge ? "nnd_update_merge" : "nnd_update");
+      // the last iteration writes the lists with the old flags stripped
+      // (what k_nnd_finalize does otherwise)
+      const int flags = (it == 0 && first_full ? kUpdFull : 0) | (it == iterations - 1 ? kUpdStrip : 0);
       hipLaunchKernelGGL(k_nnd_update, dim3(static_cast<unsigned>(ngroups)), dim3(256), 0, ctx->stream, lists, n, bu,
-                         it == 0 && first_full ? 1 : 0);
+                         flags);
       HDB_LAUNCHED();
     }
   }
@@ -1321,7 +1328,7 @@ int nnd_run(hdb_ctx* ctx, const float* X, int64_t n, int d, const hdb_params& pr
     HDB_LAUNCHED();
   }
   HDB_CALL(iterate(ctx, X, n, d, prm.sample_num, false, 0, iterations, lists, b, init == NndInit::kRandom));
-  {
+  if (iterations == 0) {  // otherwise the last update wrote the lists without old flags
     Stage st(ctx, "nnd_finalize");
     hipLaunchKernelGGL(k_nnd_finalize, dim3(grid_for(n * kK, 256)), dim3(256), 0, ctx->stream, lists, n * kK);
     HDB_LAUNCHED();
@@ -1344,7 +1351,7 @@ int nnd_merge_pair(hdb_ctx* ctx, const float* X, int64_t n1, int64_t n2, int d, 
     HDB_LAUNCHED();
   }
   HDB_CALL(iterate(ctx, X, n, d, prm.merge_sample_num, true, n1, prm.merge_iteration, out, b));
-  {
+  if (prm.merge_iteration == 0) {  // otherwise the last update wrote the lists without old flags
     Stage st(ctx, "nnd_finalize");
     hipLaunchKernelGGL(k_nnd_finalize, dim3(grid_for(n * kK, 256)), dim3(256), 0, ctx->stream, out, n * kK);
     HDB_LAUNCHED();
